@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the graph hot path on one MI355X.
+
+Workload (R-MAT scale 22, edge factor 16, a/b/c/d = 0.57/0.19/0.19/0.05,
+seed 1, symmetrised and deduplicated, uint32 CSR, weights uniform in [1, 64]
+from seed 2; 8 sources drawn with seed 3 among vertices with degree > 0):
+one step = BFS from the 8 sources + SSSP from the 8 sources + CC + PageRank
+(d = 0.85, 20 iterations). The graph and every output stay in HBM; no host
+copy runs inside the timed region.
+
+value = GTEPS of the traversal queries: sum over the 16 BFS/SSSP queries of
+m_r (sum of out-degrees of the reached vertices, directed edges) divided by
+their wall time. PageRank iterations/s and CC time are reported beside it.
+
+roofline: the query kind with the largest summed main-kernel time is the
+dominant kernel; its algorithmic bytes (BFS 12*V_r + 8*m_r, SSSP
+12*V_r + 12*m_r, CC 16*V + 8*E, PageRank 8*E + 20*V per launch) divided by
+its HIP-event-timed launch durations give `achieved`.
+
+cpu_baseline: the oracle's restatement of the reference's frontier loop
+(oracle/oracle.c, orc_ref_loop) on the host cores, one BFS and one SSSP query
+from the first source.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "test-world_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--sources", type=int, default=8)
+    ap.add_argument("--pr-iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import ptgraph as pg
+
+    pg.init(0)
+    cores = host_cores()
+    t0 = time.time()
+    csr = pg.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, threads=cores)
+    csr.add_weights(1, 64, seed=2, threads=cores)
+    deg = np.diff(csr.offsets.astype(np.int64))
+    rng = np.random.default_rng(3)
+    srcs = [int(s) for s in rng.choice(np.nonzero(deg > 0)[0], args.sources, replace=False)]
+    log("graph V=%d E=%d generated in %.1fs; sources %s" % (csr.V, csr.E, time.time() - t0, srcs))
+
+    cfg = pg.default_config(flags=pg.FLAG_TIMING)
+    g = pg.Graph.from_csr(csr, config=cfg)
+    V, E = csr.V, csr.E
+    out = pg.DeviceBuffer(4 * V)
+
+    kinds = ("bfs", "sssp", "cc", "pagerank")
+
+    def step(acc):
+        for s in srcs:
+            t = time.perf_counter()
+            _, st = g.bfs(s, out=out)
+            acc["bfs"].append((time.perf_counter() - t, st.edges_traversed, st.reached, st.main_kernel_ns,
+                               st.main_launches, st.iterations))
+        for s in srcs:
+            t = time.perf_counter()
+            _, st = g.sssp(s, out=out)
+            acc["sssp"].append((time.perf_counter() - t, st.edges_traversed, st.reached, st.main_kernel_ns,
+                                st.main_launches, st.iterations))
+        t = time.perf_counter()
+        _, st = g.cc(out=out)
+        acc["cc"].append((time.perf_counter() - t, E, V, st.main_kernel_ns, st.main_launches, st.iterations))
+        t = time.perf_counter()
+        _, st = g.pagerank(0.85, args.pr_iters, out=out)
+        acc["pagerank"].append((time.perf_counter() - t, E, V, st.main_kernel_ns, st.main_launches,
+                                st.iterations))
+
+    for i in range(args.warmup):
+        step({k: [] for k in kinds})
+        log("warmup step %d done" % (i + 1))
+    acc = {k: [] for k in kinds}
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(acc)
+    t_total = time.perf_counter() - t_start
+    log("timed %d steps in %.3fs" % (args.steps, t_total))
+
+    def tot(k, idx):
+        return sum(r[idx] for r in acc[k])
+
+    trav_edges = tot("bfs", 1) + tot("sssp", 1)
+    trav_time = tot("bfs", 0) + tot("sssp", 0)
+    gteps = trav_edges / trav_time / 1e9
+    bfs_gteps = tot("bfs", 1) / tot("bfs", 0) / 1e9
+    sssp_gteps = tot("sssp", 1) / tot("sssp", 0) / 1e9
+    pr_iters_s = args.pr_iters * len(acc["pagerank"]) / tot("pagerank", 0)
+    cc_ms = 1e3 * tot("cc", 0) / len(acc["cc"])
+
+    # algorithmic bytes of each kind's main kernel, summed over the timed region
+    def algo_bytes(k, r):
+        _, m_r, v_r = r[0], r[1], r[2]
+        if k == "bfs":
+            return 12 * v_r + 8 * m_r
+        if k == "sssp":
+            return 12 * v_r + 12 * m_r
+        if k == "cc":
+            return 16 * V + 8 * E
+        return args.pr_iters * (8 * E + 20 * V)
+
+    per_kind = {}
+    for k in kinds:
+        ns = tot(k, 3)
+        launches = tot(k, 4)
+        b = sum(algo_bytes(k, r) for r in acc[k])
+        per_kind[k] = {
+            "main_kernel_ms": ns / 1e6,
+            "launches": launches,
+            "algo_bytes": b,
+            "achieved_gbs": b / ns if ns else 0.0,
+            "avg_launch_us": ns / launches / 1e3 if launches else 0.0,
+        }
+    dom = max(kinds, key=lambda k: per_kind[k]["main_kernel_ms"])
+    kernel_names = {"bfs": "k_expand<BfsTopDown> + k_bfs_bottom_up", "sssp": "k_expand<Relax<true>>",
+                    "cc": "k_expand<Hook>", "pagerank": "k_pr_iter"}
+    achieved = per_kind[dom]["achieved_gbs"]
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": kernel_names[dom],
+        "bytes_per_launch": int(per_kind[dom]["algo_bytes"] / max(1, per_kind[dom]["launches"])),
+        "avg_launch_us": round(per_kind[dom]["avg_launch_us"], 2),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        if not os.path.exists(oracle._LIB_PATH):
+            oracle.build()
+        s = srcs[0]
+        log("cpu baseline: reference loop, %d threads, BFS + SSSP from %d" % (cores, s))
+        _, t_b, _, _ = oracle.ref_loop_timed(oracle.BFS, csr.node_pointers, csr.cols, src=s, threads=cores)
+        _, t_s, _, _ = oracle.ref_loop_timed(oracle.SSSP, csr.node_pointers, csr.cols, csr.weights, src=s,
+                                             threads=cores)
+        m_r = acc["bfs"][0][1]
+        cpu = {
+            "value": round(2 * m_r / ((t_b + t_s) / 1e9) / 1e9, 4),
+            "unit": "GTEPS",
+            "cores": cores,
+            "kind": "port",
+            "sample": "oracle orc_ref_loop (reference label/scan/compact/relax loop) BFS + SSSP from vertex %d "
+                      "on the same RMAT-22 graph: %.2fs + %.2fs" % (s, t_b / 1e9, t_s / 1e9),
+        }
+        log("cpu baseline %.4f GTEPS" % cpu["value"])
+
+    line = {
+        "metric": "GTEPS (BFS/SSSP) & PR iters/s at RMAT-22; batched queries/s + p99 vs plain",
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT (seed 1), weights uniform [1,64] (seed 2), sources seed 3",
+        "config": {
+            "workload": "rmat%d-ef16: BFS x%d + SSSP x%d + CC + PageRank x%d per step" % (
+                args.scale, len(srcs), len(srcs), args.pr_iters),
+            "scale": args.scale,
+            "edge_factor": 16,
+            "V": V,
+            "E": E,
+            "sources": srcs,
+        },
+        "bfs_gteps": round(bfs_gteps, 4),
+        "sssp_gteps": round(sssp_gteps, 4),
+        "pagerank_iters_per_s": round(pr_iters_s, 2),
+        "cc_ms": round(cc_ms, 3),
+        "per_kernel": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in d.items()}
+                       for k, d in per_kind.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.free()
+    g.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+/*
+ * ptg.h -- C-ABI of the MI355X (gfx950) graph hot path: BFS levels, SSSP
+ * distances, connected-component labels and PageRank over a CSR graph.
+ *
+ * This header is the drop-in boundary for ptGraph's host contract
+ * (ptGraph/gpu_kernels.cuh:26-124 kernels + ptGraph/main.cu:24-56 drivers):
+ *   - CSR in the reference layout: `nodePointers` holds V entries (not V+1),
+ *     the degree of the last vertex is E - nodePointers[V-1]
+ *     (ptGraph/main.cu:257-264); columns are uint32 (`uint *edgeList`), weighted
+ *     edges are the 8-byte AoS `EdgeWithWeight` (ptGraph/globals.cuh:2-5).
+ *   - Outputs follow the reference's value conventions:
+ *       BFS  level[src] = 1, level[v] = 1 + hops, unreached 0xFFFFFFFF
+ *            (ptGraph/main.cu:1068-1076, :1745, :1793-1794);
+ *       SSSP dist[src] = 1, dist[v] = 1 + path weight, unreached = sentinel,
+ *            0xFFFFFFFE by default (ptGraph/main.cu:584-588) or 0xFFFFFFFF
+ *            (ptGraph/main.cu:742-746);
+ *       CC   label[v] = the smallest vertex id u with a path u -> v (the fixed
+ *            point of ptGraph/gpu_kernels.cu:136-150 started from
+ *            value[v] = v, ptGraph/main.cu:977-980); on a symmetric graph this
+ *            is the minimum vertex id of v's connected component.
+ *   - Errors are returned as negative codes; the library never exits
+ *     (replaces gpuErrorcheck/exit, ptGraph/main.cu:15-22).
+ *
+ * Output pointers may be host or device memory; the library detects which.
+ * A graph handle is used by one host thread at a time.
+ */
+#ifndef PTG_H
+#define PTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define PTG_API __attribute__((visibility("default")))
+#else
+#define PTG_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bit-identical to ptGraph/globals.cuh:2-5. */
+typedef struct EdgeWithWeight {
+    uint32_t toNode;
+    uint32_t weight;
+} EdgeWithWeight;
+
+/* Return codes. */
+enum {
+    PTG_OK = 0,
+    PTG_ERR_NO_DEVICE = -1,     /* no HIP device / runtime not usable */
+    PTG_ERR_INVALID_ARG = -2,   /* null pointer, bad size, source >= V ... */
+    PTG_ERR_INVALID_GRAPH = -3, /* CSR fails validation */
+    PTG_ERR_OUT_OF_MEMORY = -4, /* device or pinned allocation failed */
+    PTG_ERR_BUDGET = -5,        /* does not fit the configured device budget */
+    PTG_ERR_HIP = -6,           /* a HIP runtime call failed */
+    PTG_ERR_UNSUPPORTED = -7,   /* operation not available for this graph */
+    PTG_ERR_REJECTED = -8,      /* serving admission refused the request */
+    PTG_ERR_IO = -9             /* file read/write failed or bad format */
+};
+
+/* Graph-creation flags (ptg_config.flags). */
+#define PTG_FLAG_NO_VALIDATE 0x1u  /* skip the host-side CSR validation */
+#define PTG_FLAG_TIMING 0x2u       /* time every main-kernel launch with HIP events */
+
+typedef struct ptg_config {
+    uint64_t device_budget_bytes; /* 0 = whole device; cap on graph + state */
+    uint64_t chunk_bytes;         /* streamed CSR chunk size (partition path) */
+    uint32_t sssp_sentinel;       /* unreached SSSP value: 0xFFFFFFFE or 0xFFFFFFFF */
+    uint32_t grid_blocks;         /* persistent grid size in workgroups; 0 = auto */
+    uint32_t small_degree;        /* degree <= small_degree: lane per vertex */
+    uint32_t large_degree;        /* degree >  large_degree: split over workgroups */
+    float td_to_bu;               /* BFS switches to bottom-up when m_f > m_u / td_to_bu */
+    float bu_to_td;               /* BFS returns to top-down when n_f < V / bu_to_td */
+    uint32_t sssp_delta;          /* near/far split step for SSSP; 0 = auto */
+    uint32_t flags;               /* PTG_FLAG_* */
+} ptg_config;
+
+/* Per-query statistics (the reference prints `iter` and `nodeSum`,
+ * ptGraph/main.cu:1109-1113; these are their counterparts). */
+typedef struct ptg_stats {
+    uint32_t iterations;      /* frontier expansions; BFS: eccentricity(src) + 1 */
+    uint32_t reached;         /* vertices with a finite value (BFS/SSSP) or components (CC) */
+    uint64_t edges_traversed; /* m_r: sum of out-degrees of reached vertices */
+    uint64_t frontier_sum;    /* sum of frontier sizes (reference nodeSum) */
+    uint64_t kernel_ns;       /* device time of the whole query (HIP events) */
+    uint64_t host_link_bytes; /* bytes copied host -> device during the query */
+    uint64_t host_link_ns;    /* time of those copies */
+    uint64_t main_kernel_ns;  /* PTG_FLAG_TIMING: summed HIP-event time of the query's
+                                 main kernel (frontier expansion / PageRank sweep) */
+    uint32_t main_launches;   /* PTG_FLAG_TIMING: launches of that kernel */
+    uint32_t reserved;
+} ptg_stats;
+
+typedef struct ptg_graph ptg_graph;
+
+/* Host CSR with V+1 offsets (offsets[V] == E); produced by the generators and
+ * readers below, released with ptg_host_csr_free. `weights` may be NULL. */
+typedef struct ptg_host_csr {
+    uint32_t V;
+    uint64_t E;
+    uint32_t *offsets; /* V + 1 entries */
+    uint32_t *cols;    /* E entries */
+    uint32_t *weights; /* E entries or NULL */
+} ptg_host_csr;
+
+/* ---- runtime -------------------------------------------------------------- */
+PTG_API int ptg_init(int device);             /* PTG_ERR_NO_DEVICE without a GPU */
+PTG_API const char *ptg_strerror(int code);
+PTG_API const char *ptg_last_error(void);     /* detail of the last failure on this thread */
+PTG_API void ptg_config_default(ptg_config *cfg);
+PTG_API int ptg_device_info(int *num_cus, uint64_t *total_mem, uint64_t *free_mem);
+
+/* Device buffers for callers that keep outputs resident in HBM. */
+PTG_API int ptg_device_alloc(uint64_t bytes, void **ptr);
+PTG_API int ptg_device_free(void *ptr);
+PTG_API int ptg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+PTG_API int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+
+/* ---- graph ---------------------------------------------------------------- */
+/* Checks the reference-layout CSR on the host: offsets non-decreasing,
+ * nodePointers[V-1] <= E, every column < V, E <= UINT32_MAX. No GPU needed. */
+PTG_API int ptg_validate_csr(uint32_t V, uint64_t E, const uint32_t *nodePointers,
+                     const uint32_t *edgeList, const EdgeWithWeight *wedges);
+
+/* Replaces the per-call setup of every ptGraph driver (degree pass
+ * main.cu:257-264, H2D copies main.cu:1712-1714 / :1272-1275). Exactly one of
+ * edgeList / wedges may be NULL; with wedges the graph carries weights. */
+PTG_API int ptg_graph_create(uint32_t V, uint64_t E, const uint32_t *nodePointers,
+                     const uint32_t *edgeList, const EdgeWithWeight *wedges,
+                     const ptg_config *cfg, ptg_graph **out);
+/* Same, from separate column and weight arrays (weights may be NULL). */
+PTG_API int ptg_graph_create_soa(uint32_t V, uint64_t E, const uint32_t *nodePointers,
+                         const uint32_t *cols, const uint32_t *weights,
+                         const ptg_config *cfg, ptg_graph **out);
+PTG_API void ptg_graph_destroy(ptg_graph *g);
+/* Device bytes ptg_graph_create would allocate for this size (CSR + query
+ * state); graph_create fails with PTG_ERR_BUDGET before allocating anything
+ * when this exceeds cfg->device_budget_bytes. */
+PTG_API uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const ptg_config *cfg);
+PTG_API uint64_t ptg_graph_device_bytes(const ptg_graph *g);
+PTG_API int ptg_graph_is_symmetric(ptg_graph *g); /* 1, 0, or negative error */
+
+/* ---- queries ---------------------------------------------------------------- */
+/* bfs_kernel / bfsCaculateIn* (gpu_kernels.cu:118-134, main.cu:1041-1123,
+ * :1180-1418, :1420-1678, :1681-2059). */
+PTG_API int ptg_bfs(ptg_graph *g, uint32_t src, uint32_t *levels_out, ptg_stats *st);
+/* sssp_kernel / ssspCaculateIn* (gpu_kernels.cu:152-172, main.cu:563-638,
+ * :678-889). Needs a weighted graph. */
+PTG_API int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *st);
+/* cc_kernel / ccCaculateIn* (gpu_kernels.cu:136-150, main.cu:961-1035, :222-440). */
+PTG_API int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st);
+/* prCaculateInShare (main.cu:891-959) has no arithmetic in the reference; this
+ * defines it: r0 = 1/V, r' = (1-d)/V + d * (sum_{u->v} r(u)/outdeg(u) + D/V),
+ * D = rank mass of vertices with outdeg 0; FP32 with compensated sums. */
+PTG_API int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out,
+                 ptg_stats *st);
+
+/* ---- C-ABI entry points named after the reference drivers ------------------ */
+/* Same signature and meaning as ptGraph/main.cu:24-56: build the graph, run one
+ * query, free everything, return elapsed milliseconds (or a negative code). */
+PTG_API long bfsCaculateInShare(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                        uint32_t *edgeList, uint32_t sourceNode);
+PTG_API long bfsCaculateInOpt(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                      uint32_t *edgeList, uint32_t sourceNode);
+PTG_API long bfsCaculateInAsync(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                        uint32_t *edgeList, uint32_t sourceNode);
+PTG_API long bfsCaculateInAsyncSwap(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                            uint32_t *edgeList, uint32_t sourceNode);
+PTG_API long ssspCaculateInShare(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                         EdgeWithWeight *edgeList, uint32_t sourceNode);
+PTG_API long ssspCaculateInOpt(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                       EdgeWithWeight *edgeList, uint32_t sourceNode);
+PTG_API long ccCaculateInShare(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                       uint32_t *edgeList);
+PTG_API long ccCaculateInOpt(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                     uint32_t *edgeList);
+PTG_API long prCaculateInShare(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                       uint32_t *edgeList);
+PTG_API long prCaculateInOpt(uint32_t testNumNodes, uint32_t testNumEdge, uint32_t *nodePointersI,
+                     uint32_t *edgeList);
+
+/* ---- synthetic inputs and files (host only, no GPU) ------------------------ */
+/* Undirected R-MAT: edge_factor * 2^scale samples from a counter-based
+ * SplitMix64 stream, self-loops dropped, symmetrised, duplicates removed,
+ * adjacency sorted ascending. No vertex permutation (vertex 0 is the hub). */
+PTG_API int ptg_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                 uint64_t seed, int threads, ptg_host_csr *out);
+/* Erdos-Renyi with the same sample count, cleaned the same way. */
+PTG_API int ptg_gen_er(uint32_t scale, uint32_t edge_factor, uint64_t seed, int threads,
+               ptg_host_csr *out);
+/* Symmetric weights w(u,v) = w(v,u) uniform in [lo, hi] from a hash of
+ * (min(u,v), max(u,v), seed); allocates csr->weights. */
+PTG_API int ptg_gen_weights(ptg_host_csr *csr, uint32_t lo, uint32_t hi, uint64_t seed, int threads);
+PTG_API void ptg_host_csr_free(ptg_host_csr *csr);
+
+/* .bcsr  = [u32 V][u32 E][u32 nodePointers x V][u32 col x E]      (main.cu:98-116, :2604-2614)
+ * .bwcsr = [u32 V][u32 E][u32 nodePointers x V][EdgeWithWeight x E] (main.cu:452-471, :2575-2602)
+ * Java/WebGraph export = big-endian [u32 V][u64 E][u64 ptr x V][u32 col x E] (main.cu:2479-2573). */
+PTG_API int ptg_read_bcsr(const char *path, ptg_host_csr *out);
+PTG_API int ptg_write_bcsr(const char *path, const ptg_host_csr *csr);
+PTG_API int ptg_read_bwcsr(const char *path, ptg_host_csr *out);
+PTG_API int ptg_write_bwcsr(const char *path, const ptg_host_csr *csr);
+PTG_API int ptg_read_java_be(const char *path, ptg_host_csr *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTG_H */

@@ -1,0 +1,316 @@
+// PageRank for gfx950: pull over in-edges, FP32 with compensated sums.
+//
+// The reference has no PageRank arithmetic (prCaculateInShare,
+// ptGraph/main.cu:891-959, launches nothing; testPagerank runs CC at
+// ptGraph/main.cu:125). The definition used here:
+//   r0(v) = 1/V
+//   r'(v) = (1-d)/V + d * ( sum_{u->v} r(u)/outdeg(u) + D/V ),
+//   D     = sum of r(u) over vertices with outdeg(u) = 0.
+// One kernel per iteration over static in-degree bins (hub lists split into
+// chunks whose partials are combined in chunk order by the last chunk to
+// finish). Every sum is a (hi, lo) pair updated with Knuth's TwoSum, and every
+// reduction runs in a fixed order, so results are bit-identical run to run.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "device_util.h"
+
+namespace ptg {
+
+struct CF {
+    float hi, lo;
+};
+
+__device__ __forceinline__ void cf_add(CF &a, float x) {
+    const float s = a.hi + x;
+    const float bb = s - a.hi;
+    const float err = (a.hi - (s - bb)) + (x - bb);
+    a.hi = s;
+    a.lo += err;
+}
+
+__device__ __forceinline__ CF cf_merge(CF a, CF b) {
+    const float s = a.hi + b.hi;
+    const float bb = s - a.hi;
+    const float err = (a.hi - (s - bb)) + (b.hi - bb);
+    return CF{s, (a.lo + b.lo) + err};
+}
+
+__device__ __forceinline__ CF wave_cf_sum(CF a) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        CF b;
+        b.hi = __shfl_xor(a.hi, d, 64);
+        b.lo = __shfl_xor(a.lo, d, 64);
+        a = cf_merge(a, b);
+    }
+    return a;
+}
+
+// Sum over the workgroup in a fixed order; every thread gets the result.
+__device__ __forceinline__ CF block_cf_sum(CF a, CF *lds) {
+    a = wave_cf_sum(a);
+    if (lane_id() == 0) lds[threadIdx.x >> 6] = a;
+    __syncthreads();
+    CF t = lds[0];
+    for (int w = 1; w < kWavesPerBlock; w++) t = cf_merge(t, lds[w]);
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ float load_coherent(const float *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct PrArgs {
+    Frontier bins;                  // in-degree bins
+    const uint32_t *zero_list;      // vertices with in-degree 0
+    uint32_t n_zero;
+    const uint32_t *ioff, *icol;    // in-edges
+    const uint32_t *ooff;           // out offsets (outdeg)
+    const float *cin;               // r(u)/outdeg(u) of the previous iteration
+    float *cout;                    // same for this iteration
+    float *rank_out;                // ranks (last iteration only) or null
+    const float *dang_in;           // (hi, lo) of D for the previous iteration
+    float *dang_out;
+    float *part;                    // per-workgroup (hi, lo)
+    uint32_t *sync;                 // workgroup completion counter
+    float *lpart;                   // hub chunk partials (hi, lo)
+    uint32_t *lcnt;                 // hub chunk completion counters
+    float damping, inv_v;
+};
+
+__global__ __launch_bounds__(kBlock) void k_pr_iter(PrArgs a) {
+    __shared__ CF lds[kWavesPerBlock];
+    __shared__ uint32_t is_last;
+    const float dang = a.dang_in[0] + a.dang_in[1];
+    const float base = (1.0f - a.damping) * a.inv_v + a.damping * (dang * a.inv_v);
+    CF my_dang{0.0f, 0.0f};
+    auto finalize = [&](uint32_t v, float sum) {
+        const float r = base + a.damping * sum;
+        const uint32_t od = a.ooff[v + 1] - a.ooff[v];
+        a.cout[v] = od ? r / (float)od : 0.0f;
+        if (a.rank_out) a.rank_out[v] = r;
+        if (!od) cf_add(my_dang, r);
+    };
+    const uint32_t lane = lane_id();
+
+    // Hubs: one workgroup per chunk of kChunkEdges in-edges.
+    const unsigned long long lp = a.bins.ctr->l_pack;
+    const uint32_t nl = (uint32_t)(lp >> kLCountShift);
+    const uint32_t nch = (uint32_t)(lp & kLChunkMask);
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        uint32_t lo = 0, hi = nl;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.bins.lb[mid] <= c) lo = mid; else hi = mid;
+        }
+        const uint32_t v = a.bins.ql[lo];
+        const uint32_t first = a.bins.lb[lo];
+        const uint64_t vb = a.ioff[v], ve = a.ioff[v + 1];
+        const uint64_t b = vb + (uint64_t)(c - first) * kChunkEdges;
+        const uint64_t e = min(ve, b + kChunkEdges);
+        CF acc{0.0f, 0.0f};
+        for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) cf_add(acc, a.cin[a.icol[i]]);
+        acc = block_cf_sum(acc, lds);
+        if (threadIdx.x == 0) {
+            a.lpart[2ull * c] = acc.hi;
+            a.lpart[2ull * c + 1] = acc.lo;
+            __threadfence();
+            const uint32_t nchv = (uint32_t)((ve - vb + kChunkEdges - 1) / kChunkEdges);
+            if (atomicAdd(&a.lcnt[lo], 1u) == nchv - 1) {
+                __threadfence();
+                CF t{0.0f, 0.0f};
+                for (uint32_t j = 0; j < nchv; j++)
+                    t = cf_merge(t, CF{load_coherent(&a.lpart[2ull * (first + j)]),
+                                       load_coherent(&a.lpart[2ull * (first + j) + 1])});
+                finalize(v, t.hi + t.lo);
+                a.lcnt[lo] = 0u;
+            }
+        }
+    }
+
+    // Medium in-degree: one wave per vertex.
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const uint32_t nm = a.bins.ctr->n_m;
+    for (uint32_t i = gw; i < nm; i += nw) {
+        const uint32_t v = a.bins.qm[i];
+        const uint64_t b = a.ioff[v], e = a.ioff[v + 1];
+        CF acc{0.0f, 0.0f};
+        for (uint64_t k = b + lane; k < e; k += 64) cf_add(acc, a.cin[a.icol[k]]);
+        acc = wave_cf_sum(acc);
+        if (lane == 0) finalize(v, acc.hi + acc.lo);
+    }
+
+    // Small in-degree: one lane per vertex; then in-degree 0.
+    const uint32_t gt = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t nt = gridDim.x * kBlock;
+    const uint32_t ns = a.bins.ctr->n_s;
+    for (uint32_t i = gt; i < ns; i += nt) {
+        const uint32_t v = a.bins.qs[i];
+        const uint32_t b = a.ioff[v], e = a.ioff[v + 1];
+        CF acc{0.0f, 0.0f};
+        for (uint32_t k = b; k < e; k++) cf_add(acc, a.cin[a.icol[k]]);
+        finalize(v, acc.hi + acc.lo);
+    }
+    for (uint32_t i = gt; i < a.n_zero; i += nt) finalize(a.zero_list[i], 0.0f);
+
+    // Dangling mass: per-workgroup partials, summed in index order by the
+    // last workgroup to finish.
+    const CF bd = block_cf_sum(my_dang, lds);
+    if (threadIdx.x == 0) {
+        a.part[2 * blockIdx.x] = bd.hi;
+        a.part[2 * blockIdx.x + 1] = bd.lo;
+        __threadfence();
+        is_last = atomicAdd(&a.sync[0], 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        CF t{0.0f, 0.0f};
+        for (uint32_t j = threadIdx.x; j < gridDim.x; j += kBlock)
+            t = cf_merge(t, CF{load_coherent(&a.part[2 * j]), load_coherent(&a.part[2 * j + 1])});
+        t = block_cf_sum(t, lds);
+        if (threadIdx.x == 0) {
+            a.dang_out[0] = t.hi;
+            a.dang_out[1] = t.lo;
+            a.sync[0] = 0u;
+        }
+    }
+}
+
+// r0 = 1/V: contrib0(v) = (1/V)/outdeg(v), or 0 for a dangling vertex.
+__global__ __launch_bounds__(kBlock) void k_pr_init(uint32_t V, const uint32_t *__restrict__ ooff, float r0,
+                                                    float *__restrict__ contrib) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        const uint32_t od = ooff[v + 1] - ooff[v];
+        contrib[v] = od ? r0 / (float)od : 0.0f;
+    }
+}
+
+__global__ void k_set_f32x2(float *p, float x, float y) {
+    if (threadIdx.x == 0) {
+        p[0] = x;
+        p[1] = y;
+    }
+}
+
+// Transposed CSR on the host (counting sort by destination, sources visited in
+// ascending order, so every in-list is sorted and the layout is deterministic).
+int ensure_in_edges(ptg_graph *g) {
+    if (g->have_in) return PTG_OK;
+    int sym = 0;
+    int rc = check_symmetric(g, &sym);
+    if (rc) return rc;
+    if (sym) {
+        g->in = g->out;
+        g->in_is_out = true;
+        g->have_in = true;
+        return PTG_OK;
+    }
+    const uint32_t V = g->V;
+    const uint64_t E = g->E;
+    std::vector<uint32_t> off((size_t)V + 1), col(E ? E : 1), toff((size_t)V + 1, 0), tcol(E ? E : 1);
+    PTG_HIP(hipMemcpy(off.data(), g->out.off, sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyDeviceToHost));
+    if (E) PTG_HIP(hipMemcpy(col.data(), g->out.col, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+    for (uint64_t e = 0; e < E; e++) toff[col[e] + 1]++;
+    for (uint32_t v = 0; v < V; v++) toff[v + 1] += toff[v];
+    std::vector<uint32_t> cur(toff.begin(), toff.end() - 1);
+    for (uint32_t u = 0; u < V; u++)
+        for (uint32_t k = off[u]; k < off[u + 1]; k++) tcol[cur[col[k]]++] = u;
+    if ((rc = galloc(g, (void **)&g->in.off, sizeof(uint32_t) * ((uint64_t)V + 1))) ||
+        (rc = galloc(g, (void **)&g->in.col, sizeof(uint32_t) * E)))
+        return rc;
+    PTG_HIP(hipMemcpy(g->in.off, toff.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice));
+    if (E) PTG_HIP(hipMemcpy(g->in.col, tcol.data(), sizeof(uint32_t) * E, hipMemcpyHostToDevice));
+    if ((rc = build_static_bins(g, toff.data(), V, &g->in.bins))) return rc;
+    g->in_is_out = false;
+    g->have_in = true;
+    return PTG_OK;
+}
+
+static int ensure_pr_state(ptg_graph *g) {
+    if (g->d_contrib[0]) return PTG_OK;
+    int rc;
+    const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
+    const uint64_t lents = g->in.bins.h_ctr.l_pack >> kLCountShift;
+    if ((rc = galloc(g, (void **)&g->d_contrib[0], sizeof(float) * g->V)) ||
+        (rc = galloc(g, (void **)&g->d_contrib[1], sizeof(float) * g->V)) ||
+        (rc = galloc(g, (void **)&g->d_dang, sizeof(float) * 4)) ||
+        (rc = galloc(g, (void **)&g->d_part, sizeof(float) * 2 * g->grid)) ||
+        (rc = galloc(g, (void **)&g->d_sync, sizeof(uint32_t) * 4)) ||
+        (rc = galloc(g, (void **)&g->d_lpart, sizeof(float) * 2 * (chunks + 1))) ||
+        (rc = galloc(g, (void **)&g->d_lcnt, sizeof(uint32_t) * (lents + 1))))
+        return rc;
+    g->lchunks_cap = (uint32_t)chunks;
+    g->lcnt_cap = (uint32_t)lents;
+    if ((rc = fill_u32(g, g->d_sync, 4, 0u)) || (rc = fill_u32(g, g->d_lcnt, lents + 1, 0u))) return rc;
+    return PTG_OK;
+}
+
+int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
+    int rc;
+    if ((rc = ensure_in_edges(g)) || (rc = ensure_pr_state(g))) return rc;
+    const uint32_t V = g->V;
+    const float r0 = 1.0f / (float)V;
+    LaunchTimer timer(g);
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if (iters == 0) {
+        union {
+            float f;
+            uint32_t u;
+        } cv;
+        cv.f = r0;
+        if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(ranks), V, cv.u))) return rc;
+    } else {
+        const uint32_t blocks = std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid);
+        k_pr_init<<<blocks, kBlock, 0, g->stream>>>(V, g->out.off, r0, g->d_contrib[0]);
+        // D0 = (number of dangling vertices) / V, split into (hi, lo).
+        const double d0 = (double)g->out.bins.n_zero / (double)V;
+        const float d0h = (float)d0;
+        k_set_f32x2<<<1, 64, 0, g->stream>>>(g->d_dang, d0h, (float)(d0 - (double)d0h));
+        PTG_HIP(hipGetLastError());
+        for (int t = 0; t < iters; t++) {
+            PrArgs a;
+            a.bins = g->in.bins.q.with(g->in.bins.d_ctr);
+            a.zero_list = g->in.bins.zero;
+            a.n_zero = g->in.bins.n_zero;
+            a.ioff = g->in.off;
+            a.icol = g->in.col;
+            a.ooff = g->out.off;
+            a.cin = g->d_contrib[t & 1];
+            a.cout = g->d_contrib[(t + 1) & 1];
+            a.rank_out = (t == iters - 1) ? ranks : nullptr;
+            a.dang_in = g->d_dang + 2 * (t & 1);
+            a.dang_out = g->d_dang + 2 * ((t + 1) & 1);
+            a.part = g->d_part;
+            a.sync = g->d_sync;
+            a.lpart = g->d_lpart;
+            a.lcnt = g->d_lcnt;
+            a.damping = damping;
+            a.inv_v = r0;
+            timer.before();
+            k_pr_iter<<<g->grid, kBlock, 0, g->stream>>>(a);
+            timer.after();
+            PTG_HIP(hipGetLastError());
+        }
+    }
+    PTG_HIP(hipEventRecord(g->ev1, g->stream));
+    PTG_HIP(hipEventSynchronize(g->ev1));
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        st->kernel_ns = (uint64_t)((double)ms * 1e6);
+        st->iterations = (uint32_t)iters;
+        st->reached = V;
+        st->edges_traversed = g->E * (uint64_t)iters;
+        if ((rc = timer.collect(st))) return rc;
+    }
+    return PTG_OK;
+}
+
+}  // namespace ptg

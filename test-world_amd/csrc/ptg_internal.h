@@ -1,0 +1,162 @@
+// Internal declarations shared by the host runtime and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "ptg.h"
+
+namespace ptg {
+
+// Records a failure detail for ptg_last_error() and returns `code`.
+int fail(int code, const std::string &msg);
+// Converts a HIP status into PTG_ERR_HIP with a message naming `what`.
+int hip_fail(hipError_t e, const char *what);
+
+#define PTG_HIP(call)                                            \
+    do {                                                         \
+        hipError_t _e = (call);                                  \
+        if (_e != hipSuccess) return ::ptg::hip_fail(_e, #call); \
+    } while (0)
+
+constexpr int kBlock = 256;  // threads per workgroup (4 waves of 64)
+constexpr int kWavesPerBlock = kBlock / 64;
+constexpr uint32_t kChunkEdges = 2048;  // edges per workgroup task for hub vertices
+constexpr uint32_t kRing = 4096;        // per-round counter blocks kept on the device
+
+// Counters of one frontier. Bins: S = lane per vertex, M = wave per vertex,
+// L = edge chunks of kChunkEdges spread over workgroups.
+struct BinCounters {
+    uint32_t n_s;                 // entries in the S queue
+    uint32_t n_m;                 // entries in the M queue
+    unsigned long long l_pack;    // L queue: (entries << 40) | total chunks
+    uint32_t found;               // vertices discovered / improved (any degree)
+    uint32_t pad;
+    unsigned long long edges;     // sum of out-degrees of `found`
+    unsigned long long aux;       // algorithm specific
+};
+static_assert(sizeof(BinCounters) == 40, "BinCounters layout");
+
+constexpr unsigned long long kLCountShift = 40;
+constexpr unsigned long long kLChunkMask = (1ull << kLCountShift) - 1;
+
+inline uint32_t l_entries(const BinCounters &c) { return (uint32_t)(c.l_pack >> kLCountShift); }
+inline uint64_t bin_total(const BinCounters &c) { return (uint64_t)c.n_s + c.n_m + l_entries(c); }
+
+// Queues of one frontier plus the counters they are filled against.
+struct Frontier {
+    uint32_t *qs;  // S queue (vertex ids)
+    uint32_t *qm;  // M queue
+    uint32_t *ql;  // L queue vertex ids
+    uint32_t *lb;  // L queue first-chunk indices (non-decreasing with the slot)
+    BinCounters *ctr;
+};
+
+// Queue storage without counters.
+struct QueueSet {
+    uint32_t *qs, *qm, *ql, *lb;
+    Frontier with(BinCounters *c) const { return Frontier{qs, qm, ql, lb, c}; }
+};
+
+// Vertices of a CSR binned by degree once (CC hooking, PageRank pull).
+struct StaticBins {
+    QueueSet q{};
+    uint32_t *zero = nullptr;     // degree-0 vertices
+    BinCounters *d_ctr = nullptr; // counts of the bins on the device
+    BinCounters h_ctr{};
+    uint32_t n_zero = 0;
+    uint32_t *mem = nullptr;
+};
+
+struct Csr {
+    uint32_t *off = nullptr;  // V + 1
+    uint32_t *col = nullptr;  // E
+    StaticBins bins;
+};
+
+}  // namespace ptg
+
+// The graph handle.
+struct ptg_graph {
+    uint32_t V = 0;
+    uint64_t E = 0;
+    int device = 0;
+    int num_cus = 0;
+    ptg_config cfg{};
+    uint32_t grid = 0;  // persistent grid (workgroups)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // CSR in HBM: out-edges (+ SoA weights) and, when needed, in-edges.
+    ptg::Csr out;
+    uint32_t *d_w = nullptr;
+    ptg::Csr in;             // aliases `out` on a symmetric graph
+    bool in_is_out = false;
+    bool have_in = false;
+
+    // Query state, allocated once per graph and reused by every query.
+    uint32_t *d_val = nullptr;          // V: levels / distances / labels
+    uint32_t *d_bits[3] = {};           // bitmaps of nwords words
+    uint32_t nwords = 0;                // 2 * npairs
+    uint32_t npairs = 0;                // ceil(V / 64)
+    ptg::QueueSet q[2] = {};            // double-buffered frontier queues
+    uint32_t *d_qmem = nullptr;
+    ptg::BinCounters *d_ring = nullptr; // kRing per-round counter blocks
+    ptg::BinCounters *h_ring = nullptr; // pinned mirror
+    unsigned long long *d_acc = nullptr; // 4 accumulators for statistics kernels
+
+    // PageRank state.
+    float *d_contrib[2] = {};
+    float *d_dang = nullptr;            // 2 x (hi, lo)
+    float *d_part = nullptr;            // per-workgroup (hi, lo)
+    uint32_t *d_sync = nullptr;         // [0] block completion counter
+    float *d_lpart = nullptr;           // hub chunk partials (hi, lo)
+    uint32_t *d_lcnt = nullptr;         // hub chunk completion counters
+    uint32_t lchunks_cap = 0, lcnt_cap = 0;
+
+    // Per-launch timing of the main kernel (PTG_FLAG_TIMING).
+    std::vector<hipEvent_t> tev;  // pairs: [2i] before, [2i+1] after launch i
+    uint32_t tev_used = 0;
+
+    int symmetric = -1;  // -1 unknown, 0/1 once checked
+    uint64_t device_bytes = 0;
+    bool weighted = false;
+};
+
+namespace ptg {
+
+// Device allocation that is charged against the graph's budget.
+int galloc(ptg_graph *g, void **p, uint64_t bytes);
+void gfree(ptg_graph *g, void *p);
+bool is_device_pointer(const void *p);
+
+// Host-built degree bins uploaded to the device.
+int build_static_bins(ptg_graph *g, const uint32_t *h_off, uint32_t V, StaticBins *sb);
+void free_static_bins(ptg_graph *g, StaticBins *sb);
+
+// Kernel drivers (frontier.hip, pagerank.hip). `out` is a device pointer.
+int fill_u32(ptg_graph *g, uint32_t *p, uint64_t n, uint32_t v);
+int run_bfs(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
+int run_sssp(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
+int run_cc(ptg_graph *g, uint32_t *out, ptg_stats *st);
+int run_pagerank(ptg_graph *g, float damping, int iters, float *out, ptg_stats *st);
+int check_symmetric(ptg_graph *g, int *sym);
+int ensure_in_edges(ptg_graph *g);
+int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,
+                uint64_t *edges);
+
+// Brackets main-kernel launches with HIP events on the graph's stream when
+// PTG_FLAG_TIMING is set; collect() sums the pairs after the stream is idle.
+struct LaunchTimer {
+    ptg_graph *g;
+    bool on;
+    explicit LaunchTimer(ptg_graph *gr);
+    void before();
+    void after();
+    int collect(ptg_stats *st);
+};
+
+}  // namespace ptg

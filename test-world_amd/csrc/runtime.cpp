@@ -1,0 +1,467 @@
+// Host runtime: errors, device selection, buffers, and the graph handle.
+// Replaces the per-call allocate/copy/free of every ptGraph driver
+// (e.g. ptGraph/main.cu:1683-1830) with one resident handle per graph.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ptg_internal.h"
+
+namespace ptg {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return PTG_ERR_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return PTG_ERR_NO_DEVICE;
+    return PTG_ERR_HIP;
+}
+
+bool is_device_pointer(const void *p) {
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+int galloc(ptg_graph *g, void **p, uint64_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    if (g->cfg.device_budget_bytes && g->device_bytes + bytes > g->cfg.device_budget_bytes)
+        return fail(PTG_ERR_BUDGET, "device budget of " + std::to_string(g->cfg.device_budget_bytes) +
+                                        " bytes exceeded (need " + std::to_string(g->device_bytes + bytes) + ")");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    g->device_bytes += bytes;
+    return PTG_OK;
+}
+
+void gfree(ptg_graph *, void *p) {
+    if (p) (void)hipFree(p);
+}
+
+int build_static_bins(ptg_graph *g, const uint32_t *h_off, uint32_t V, StaticBins *sb) {
+    const uint32_t sd = g->cfg.small_degree, ld = g->cfg.large_degree;
+    std::vector<uint32_t> s, m, l, lb, z;
+    uint64_t chunks = 0;
+    for (uint32_t v = 0; v < V; v++) {
+        uint32_t d = h_off[v + 1] - h_off[v];
+        if (d == 0)
+            z.push_back(v);
+        else if (d <= sd)
+            s.push_back(v);
+        else if (d <= ld)
+            m.push_back(v);
+        else {
+            l.push_back(v);
+            lb.push_back((uint32_t)chunks);
+            chunks += (d + kChunkEdges - 1) / kChunkEdges;
+        }
+    }
+    if (chunks > 0xFFFFFFFFull) return fail(PTG_ERR_UNSUPPORTED, "too many hub chunks");
+    // five lists, each followed by one pad entry, then 8-byte aligned counters
+    const uint64_t n = (uint64_t)s.size() + m.size() + 2 * l.size() + z.size() + 5;
+    int rc = galloc(g, (void **)&sb->mem, n * sizeof(uint32_t) + 8 + sizeof(BinCounters));
+    if (rc) return rc;
+    uint32_t *p = sb->mem;
+    auto put = [&](std::vector<uint32_t> &src, uint32_t *&dst) -> int {
+        dst = p;
+        if (!src.empty()) {
+            hipError_t e = hipMemcpy(p, src.data(), src.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpy(static bins)");
+        }
+        p += src.size() + 1;
+        return PTG_OK;
+    };
+    if ((rc = put(s, sb->q.qs)) || (rc = put(m, sb->q.qm)) || (rc = put(l, sb->q.ql)) ||
+        (rc = put(lb, sb->q.lb)) || (rc = put(z, sb->zero)))
+        return rc;
+    // counters live after the lists (aligned to 8 bytes)
+    uintptr_t cp = ((uintptr_t)p + 7) & ~(uintptr_t)7;
+    sb->d_ctr = (BinCounters *)cp;
+    memset(&sb->h_ctr, 0, sizeof sb->h_ctr);
+    sb->h_ctr.n_s = (uint32_t)s.size();
+    sb->h_ctr.n_m = (uint32_t)m.size();
+    sb->h_ctr.l_pack = ((unsigned long long)l.size() << kLCountShift) | chunks;
+    sb->h_ctr.found = (uint32_t)(s.size() + m.size() + l.size());
+    sb->n_zero = (uint32_t)z.size();
+    PTG_HIP(hipMemcpy(sb->d_ctr, &sb->h_ctr, sizeof(BinCounters), hipMemcpyHostToDevice));
+    return PTG_OK;
+}
+
+void free_static_bins(ptg_graph *g, StaticBins *sb) {
+    gfree(g, sb->mem);
+    *sb = StaticBins{};
+}
+
+LaunchTimer::LaunchTimer(ptg_graph *gr) : g(gr), on((gr->cfg.flags & PTG_FLAG_TIMING) != 0) {
+    g->tev_used = 0;
+}
+
+void LaunchTimer::before() {
+    if (!on) return;
+    const size_t need = 2ull * (g->tev_used + 1);
+    if (g->tev.size() < need) {
+        if (g->tev.size() >= 2ull * 65536) {  // cap: stop timing very long queries
+            on = false;
+            return;
+        }
+        while (g->tev.size() < need) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) {
+                on = false;
+                return;
+            }
+            g->tev.push_back(e);
+        }
+    }
+    (void)hipEventRecord(g->tev[2 * g->tev_used], g->stream);
+}
+
+void LaunchTimer::after() {
+    if (!on) return;
+    (void)hipEventRecord(g->tev[2 * g->tev_used + 1], g->stream);
+    g->tev_used++;
+}
+
+int LaunchTimer::collect(ptg_stats *st) {
+    if (!st) return PTG_OK;
+    double ns = 0;
+    for (uint32_t i = 0; i < g->tev_used; i++) {
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, g->tev[2 * i], g->tev[2 * i + 1]));
+        ns += (double)ms * 1e6;
+    }
+    st->main_kernel_ns = (uint64_t)ns;
+    st->main_launches = g->tev_used;
+    return PTG_OK;
+}
+
+static void destroy(ptg_graph *g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    gfree(g, g->out.off);
+    gfree(g, g->out.col);
+    gfree(g, g->d_w);
+    free_static_bins(g, &g->out.bins);
+    if (g->have_in && !g->in_is_out) {
+        gfree(g, g->in.off);
+        gfree(g, g->in.col);
+        free_static_bins(g, &g->in.bins);
+    }
+    gfree(g, g->d_val);
+    for (auto *b : g->d_bits) gfree(g, b);
+    gfree(g, g->d_qmem);
+    gfree(g, g->d_ring);
+    gfree(g, g->d_acc);
+    for (auto *c : g->d_contrib) gfree(g, c);
+    gfree(g, g->d_dang);
+    gfree(g, g->d_part);
+    gfree(g, g->d_sync);
+    gfree(g, g->d_lpart);
+    gfree(g, g->d_lcnt);
+    if (g->h_ring) (void)hipHostFree(g->h_ring);
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    for (auto e : g->tev) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols,
+                       const uint32_t *weights, const EdgeWithWeight *wedges, const ptg_config *cfg,
+                       ptg_graph **out) {
+    if (!out) return fail(PTG_ERR_INVALID_ARG, "graph_create: out is null");
+    *out = nullptr;
+    ptg_config c;
+    if (cfg)
+        c = *cfg;
+    else
+        ptg_config_default(&c);
+    if (c.small_degree == 0 || c.large_degree < c.small_degree || c.large_degree < 64)
+        return fail(PTG_ERR_INVALID_ARG, "graph_create: need 0 < small_degree <= large_degree, large_degree >= 64");
+    if (c.sssp_sentinel == 0) c.sssp_sentinel = 0xFFFFFFFEu;
+    if (!(c.flags & PTG_FLAG_NO_VALIDATE)) {
+        int rc = ptg_validate_csr(V, E, ptr, cols, wedges);
+        if (rc) return rc;
+    } else if (V == 0 || !ptr || E > 0xFFFFFFFFull || (E && !cols && !wedges)) {
+        return fail(PTG_ERR_INVALID_ARG, "graph_create: bad sizes or null arrays");
+    }
+    const bool weighted = (weights != nullptr) || (wedges != nullptr);
+    const uint64_t need = ptg_graph_estimate_bytes(V, E, weighted, &c);
+    if (c.device_budget_bytes && need > c.device_budget_bytes)
+        return fail(PTG_ERR_BUDGET, "graph needs " + std::to_string(need) + " device bytes, budget is " +
+                                        std::to_string(c.device_budget_bytes));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(PTG_ERR_NO_DEVICE, "no HIP device available");
+    }
+    ptg_graph *g = new ptg_graph();
+    g->V = V;
+    g->E = E;
+    g->cfg = c;
+    g->weighted = weighted;
+    int rc = PTG_OK;
+#define TRY(x)                 \
+    do {                       \
+        rc = (x);              \
+        if (rc) {              \
+            destroy(g);        \
+            return rc;         \
+        }                      \
+    } while (0)
+    TRY(hipGetDevice(&g->device) == hipSuccess ? PTG_OK : fail(PTG_ERR_NO_DEVICE, "hipGetDevice failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, g->device) != hipSuccess) TRY(fail(PTG_ERR_HIP, "hipGetDeviceProperties"));
+    g->num_cus = prop.multiProcessorCount;
+    g->grid = c.grid_blocks ? c.grid_blocks : (uint32_t)g->num_cus * 8u;
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        TRY(fail(PTG_ERR_HIP, "hipStreamCreateWithFlags"));
+    if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
+        TRY(fail(PTG_ERR_HIP, "hipEventCreate"));
+
+    // CSR: V+1 offsets built from the reference's V-entry layout.
+    std::vector<uint32_t> h_off((size_t)V + 1);
+    memcpy(h_off.data(), ptr, sizeof(uint32_t) * V);
+    h_off[V] = (uint32_t)E;
+    TRY(galloc(g, (void **)&g->out.off, sizeof(uint32_t) * ((uint64_t)V + 1)));
+    TRY(galloc(g, (void **)&g->out.col, sizeof(uint32_t) * E));
+    if (g->weighted) TRY(galloc(g, (void **)&g->d_w, sizeof(uint32_t) * E));
+    if (hipMemcpy(g->out.off, h_off.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice) != hipSuccess)
+        TRY(fail(PTG_ERR_HIP, "hipMemcpy(offsets)"));
+    if (cols) {
+        if (E && hipMemcpy(g->out.col, cols, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
+            TRY(fail(PTG_ERR_HIP, "hipMemcpy(cols)"));
+        if (weights && E && hipMemcpy(g->d_w, weights, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
+            TRY(fail(PTG_ERR_HIP, "hipMemcpy(weights)"));
+    } else if (E) {
+        // AoS EdgeWithWeight -> SoA columns + weights, in host blocks.
+        const uint64_t blk = 1ull << 22;
+        std::vector<uint32_t> tc(std::min<uint64_t>(blk, E)), tw(std::min<uint64_t>(blk, E));
+        for (uint64_t e0 = 0; e0 < E; e0 += blk) {
+            uint64_t n = std::min(blk, E - e0);
+            for (uint64_t i = 0; i < n; i++) {
+                tc[i] = wedges[e0 + i].toNode;
+                tw[i] = wedges[e0 + i].weight;
+            }
+            if (hipMemcpy(g->out.col + e0, tc.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(g->d_w + e0, tw.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess)
+                TRY(fail(PTG_ERR_HIP, "hipMemcpy(edges)"));
+        }
+    }
+    TRY(build_static_bins(g, h_off.data(), V, &g->out.bins));
+
+    // Query state.
+    g->npairs = (V + 63) / 64;
+    g->nwords = 2 * g->npairs;
+    TRY(galloc(g, (void **)&g->d_val, sizeof(uint32_t) * V));
+    for (auto &b : g->d_bits) TRY(galloc(g, (void **)&b, sizeof(uint32_t) * g->nwords));
+    const uint64_t lcap = std::min<uint64_t>(V, E / (c.large_degree + 1) + 1);
+    const uint64_t per_set = 2ull * V + 2 * lcap + 8;
+    TRY(galloc(g, (void **)&g->d_qmem, sizeof(uint32_t) * 2 * per_set));
+    for (int i = 0; i < 2; i++) {
+        uint32_t *p = g->d_qmem + i * per_set;
+        g->q[i].qs = p;
+        g->q[i].qm = p + V;
+        g->q[i].ql = p + 2ull * V;
+        g->q[i].lb = p + 2ull * V + lcap;
+    }
+    TRY(galloc(g, (void **)&g->d_ring, sizeof(BinCounters) * kRing));
+    TRY(galloc(g, (void **)&g->d_acc, sizeof(unsigned long long) * 4));
+    if (hipHostMalloc((void **)&g->h_ring, sizeof(BinCounters) * kRing, hipHostMallocDefault) != hipSuccess)
+        TRY(fail(PTG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters)"));
+#undef TRY
+    *out = g;
+    return PTG_OK;
+}
+
+}  // namespace ptg
+
+using namespace ptg;
+
+// Runs `fn` into a device buffer: the caller's when it is device memory,
+// otherwise the graph's scratch followed by a copy back to the host.
+template <class T, class F>
+static int with_output(ptg_graph *g, T *user, uint64_t n, T *scratch, F fn) {
+    if (!user) return fail(PTG_ERR_INVALID_ARG, "null output pointer");
+    if (is_device_pointer(user)) return fn(user);
+    int rc = fn(scratch);
+    if (rc) return rc;
+    PTG_HIP(hipMemcpyAsync(user, scratch, n * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    return PTG_OK;
+}
+
+extern "C" {
+
+const char *ptg_strerror(int code) {
+    switch (code) {
+        case PTG_OK: return "success";
+        case PTG_ERR_NO_DEVICE: return "no usable HIP device";
+        case PTG_ERR_INVALID_ARG: return "invalid argument";
+        case PTG_ERR_INVALID_GRAPH: return "invalid CSR graph";
+        case PTG_ERR_OUT_OF_MEMORY: return "out of memory";
+        case PTG_ERR_BUDGET: return "device memory budget exceeded";
+        case PTG_ERR_HIP: return "HIP runtime error";
+        case PTG_ERR_UNSUPPORTED: return "unsupported operation";
+        case PTG_ERR_REJECTED: return "request rejected by admission control";
+        case PTG_ERR_IO: return "file I/O error";
+        default: return "unknown error";
+    }
+}
+
+const char *ptg_last_error(void) { return g_last_error.c_str(); }
+
+void ptg_config_default(ptg_config *c) {
+    if (!c) return;
+    memset(c, 0, sizeof *c);
+    c->device_budget_bytes = 0;
+    c->chunk_bytes = 256ull << 20;
+    c->sssp_sentinel = 0xFFFFFFFEu;  // ssspCaculateInShare, ptGraph/main.cu:584
+    c->grid_blocks = 0;
+    c->small_degree = 32;
+    c->large_degree = 4096;
+    c->td_to_bu = 14.0f;
+    c->bu_to_td = 24.0f;
+    c->sssp_delta = 0;
+    c->flags = 0;
+}
+
+int ptg_init(int device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(PTG_ERR_NO_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return fail(PTG_ERR_INVALID_ARG, "device index out of range");
+    PTG_HIP(hipSetDevice(device));
+    return PTG_OK;
+}
+
+int ptg_device_info(int *num_cus, uint64_t *total_mem, uint64_t *free_mem) {
+    int dev = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(PTG_ERR_NO_DEVICE, "no HIP device available");
+    }
+    PTG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    PTG_HIP(hipGetDeviceProperties(&p, dev));
+    size_t fr = 0, tot = 0;
+    PTG_HIP(hipMemGetInfo(&fr, &tot));
+    if (num_cus) *num_cus = p.multiProcessorCount;
+    if (total_mem) *total_mem = tot;
+    if (free_mem) *free_mem = fr;
+    return PTG_OK;
+}
+
+int ptg_device_alloc(uint64_t bytes, void **ptr) {
+    if (!ptr) return fail(PTG_ERR_INVALID_ARG, "device_alloc: null ptr");
+    PTG_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return PTG_OK;
+}
+
+int ptg_device_free(void *ptr) {
+    if (ptr) PTG_HIP(hipFree(ptr));
+    return PTG_OK;
+}
+
+int ptg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
+    PTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PTG_OK;
+}
+
+int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
+    PTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PTG_OK;
+}
+
+int ptg_graph_create(uint32_t V, uint64_t E, const uint32_t *nodePointers, const uint32_t *edgeList,
+                     const EdgeWithWeight *wedges, const ptg_config *cfg, ptg_graph **out) {
+    if (edgeList && wedges)
+        return fail(PTG_ERR_INVALID_ARG, "graph_create: pass edgeList or wedges, not both");
+    return create_impl(V, E, nodePointers, edgeList, nullptr, wedges, cfg, out);
+}
+
+int ptg_graph_create_soa(uint32_t V, uint64_t E, const uint32_t *nodePointers, const uint32_t *cols,
+                         const uint32_t *weights, const ptg_config *cfg, ptg_graph **out) {
+    if (E && !cols) return fail(PTG_ERR_INVALID_ARG, "graph_create_soa: null cols");
+    static const uint32_t dummy = 0;
+    return create_impl(V, E, nodePointers, cols ? cols : &dummy, weights, nullptr, cfg, out);
+}
+
+void ptg_graph_destroy(ptg_graph *g) { destroy(g); }
+
+uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const ptg_config *cfg) {
+    ptg_config c;
+    if (cfg)
+        c = *cfg;
+    else
+        ptg_config_default(&c);
+    const uint64_t v1 = (uint64_t)V + 1;
+    const uint64_t large = c.large_degree ? c.large_degree : 4096;
+    const uint64_t lcap = std::min<uint64_t>(V, E / (large + 1) + 1);
+    const uint64_t nwords = 2ull * ((V + 63) / 64);
+    auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+    uint64_t b = al(4 * v1) + al(4 * E) + (weighted ? al(4 * E) : 0);  // CSR
+    b += al(4 * (uint64_t)V + 4 * (2 * lcap) + 64);                    // static bins
+    b += al(4 * (uint64_t)V) + 3 * al(4 * nwords);                     // values + bitmaps
+    b += al(4 * 2 * (2ull * V + 2 * lcap + 8));                        // frontier queues
+    b += al(sizeof(BinCounters) * kRing) + al(32);                     // counter ring, accumulators
+    return b;
+}
+
+uint64_t ptg_graph_device_bytes(const ptg_graph *g) { return g ? g->device_bytes : 0; }
+
+int ptg_graph_is_symmetric(ptg_graph *g) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    int s = 0;
+    int rc = check_symmetric(g, &s);
+    return rc ? rc : s;
+}
+
+int ptg_bfs(ptg_graph *g, uint32_t src, uint32_t *levels_out, ptg_stats *st) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "bfs: source >= V");
+    return with_output(g, levels_out, g->V, g->d_val, [&](uint32_t *o) { return run_bfs(g, src, o, st); });
+}
+
+int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *st) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "sssp: source >= V");
+    if (!g->weighted) return fail(PTG_ERR_UNSUPPORTED, "sssp: graph has no weights");
+    return with_output(g, dist_out, g->V, g->d_val, [&](uint32_t *o) { return run_sssp(g, src, o, st); });
+}
+
+int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    return with_output(g, labels_out, g->V, g->d_val, [&](uint32_t *o) { return run_cc(g, o, st); });
+}
+
+int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out, ptg_stats *st) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (iterations < 0 || !(damping >= 0.0f && damping <= 1.0f))
+        return fail(PTG_ERR_INVALID_ARG, "pagerank: need iterations >= 0 and 0 <= damping <= 1");
+    return with_output(g, ranks_out, g->V, (float *)g->d_val,
+                       [&](float *o) { return run_pagerank(g, damping, iterations, o, st); });
+}
+
+}  // extern "C"

@@ -1,0 +1,419 @@
+"""Python view of libptgraph.so, the C-ABI declared in include/ptg.h.
+
+The graph path itself is C++ and HIP; this module only marshals numpy arrays
+through ctypes so that tests and the benchmark drive the same entry points a
+C/C++ caller of the reference's drivers would (ptGraph/main.cu:24-56).
+
+If libptgraph.so is missing this module raises at load time: there is no
+Python or CPU fallback for any query.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libptgraph.so")
+
+OK = 0
+ERR_NO_DEVICE = -1
+ERR_INVALID_ARG = -2
+ERR_INVALID_GRAPH = -3
+ERR_OUT_OF_MEMORY = -4
+ERR_BUDGET = -5
+ERR_HIP = -6
+ERR_UNSUPPORTED = -7
+ERR_REJECTED = -8
+ERR_IO = -9
+
+FLAG_NO_VALIDATE = 0x1
+FLAG_TIMING = 0x2
+
+BFS_UNREACHED = 0xFFFFFFFF  # ptGraph/main.cu:1068
+SSSP_SENTINEL_SHARE = 0xFFFFFFFE  # ptGraph/main.cu:584
+SSSP_SENTINEL_OPT = 0xFFFFFFFF  # ptGraph/main.cu:742
+
+
+class PtgError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        L = lib()
+        msg = "%s failed: %s (%d): %s" % (
+            where,
+            L.ptg_strerror(code).decode(),
+            code,
+            L.ptg_last_error().decode(errors="replace"),
+        )
+        super().__init__(msg)
+
+
+class EdgeWithWeight(ctypes.Structure):
+    """ptGraph/globals.cuh:2-5."""
+
+    _fields_ = [("toNode", ctypes.c_uint32), ("weight", ctypes.c_uint32)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("device_budget_bytes", ctypes.c_uint64),
+        ("chunk_bytes", ctypes.c_uint64),
+        ("sssp_sentinel", ctypes.c_uint32),
+        ("grid_blocks", ctypes.c_uint32),
+        ("small_degree", ctypes.c_uint32),
+        ("large_degree", ctypes.c_uint32),
+        ("td_to_bu", ctypes.c_float),
+        ("bu_to_td", ctypes.c_float),
+        ("sssp_delta", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("reached", ctypes.c_uint32),
+        ("edges_traversed", ctypes.c_uint64),
+        ("frontier_sum", ctypes.c_uint64),
+        ("kernel_ns", ctypes.c_uint64),
+        ("host_link_bytes", ctypes.c_uint64),
+        ("host_link_ns", ctypes.c_uint64),
+        ("main_kernel_ns", ctypes.c_uint64),
+        ("main_launches", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class _HostCsr(ctypes.Structure):
+    _fields_ = [
+        ("V", ctypes.c_uint32),
+        ("E", ctypes.c_uint64),
+        ("offsets", ctypes.POINTER(ctypes.c_uint32)),
+        ("cols", ctypes.POINTER(ctypes.c_uint32)),
+        ("weights", ctypes.POINTER(ctypes.c_uint32)),
+    ]
+
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_lib = None
+
+# Every exported function of include/ptg.h: name -> (restype, argtypes).
+SIGNATURES = {
+    "ptg_init": (ctypes.c_int, [ctypes.c_int]),
+    "ptg_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "ptg_last_error": (ctypes.c_char_p, []),
+    "ptg_config_default": (None, [ctypes.POINTER(Config)]),
+    "ptg_device_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.POINTER(ctypes.c_uint64)]),
+    "ptg_device_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "ptg_device_free": (ctypes.c_int, [_vp]),
+    "ptg_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    "ptg_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    "ptg_validate_csr": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p,
+                                        ctypes.POINTER(EdgeWithWeight)]),
+    "ptg_graph_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p,
+                                        ctypes.POINTER(EdgeWithWeight), ctypes.POINTER(Config),
+                                        ctypes.POINTER(_vp)]),
+    "ptg_graph_create_soa": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p, _u32p,
+                                            ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
+    "ptg_graph_destroy": (None, [_vp]),
+    "ptg_graph_estimate_bytes": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                                   ctypes.POINTER(Config)]),
+    "ptg_graph_device_bytes": (ctypes.c_uint64, [_vp]),
+    "ptg_graph_is_symmetric": (ctypes.c_int, [_vp]),
+    "ptg_bfs": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
+    "ptg_sssp": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
+    "ptg_cc": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Stats)]),
+    "ptg_pagerank": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
+    "bfsCaculateInShare": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
+    "bfsCaculateInOpt": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
+    "bfsCaculateInAsync": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
+    "bfsCaculateInAsyncSwap": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
+    "ssspCaculateInShare": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                            ctypes.POINTER(EdgeWithWeight), ctypes.c_uint32]),
+    "ssspCaculateInOpt": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                          ctypes.POINTER(EdgeWithWeight), ctypes.c_uint32]),
+    "ccCaculateInShare": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
+    "ccCaculateInOpt": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
+    "prCaculateInShare": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
+    "prCaculateInOpt": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
+    "ptg_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_HostCsr)]),
+    "ptg_gen_er": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                  ctypes.POINTER(_HostCsr)]),
+    "ptg_gen_weights": (ctypes.c_int, [ctypes.POINTER(_HostCsr), ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint64, ctypes.c_int]),
+    "ptg_host_csr_free": (None, [ctypes.POINTER(_HostCsr)]),
+    "ptg_read_bcsr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_HostCsr)]),
+    "ptg_write_bcsr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_HostCsr)]),
+    "ptg_read_bwcsr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_HostCsr)]),
+    "ptg_write_bwcsr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_HostCsr)]),
+    "ptg_read_java_be": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_HostCsr)]),
+}
+
+
+def lib():
+    """Loads libptgraph.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libptgraph.so is not built: run __graft_entry__.build() "
+                               "or `make -C test-world_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, where):
+    if rc != OK:
+        raise PtgError(rc, where)
+
+
+def _u32(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def init(device=0):
+    _check(lib().ptg_init(device), "ptg_init")
+
+
+def default_config(**overrides):
+    c = Config()
+    lib().ptg_config_default(ctypes.byref(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+def device_info():
+    n = ctypes.c_int(0)
+    tot, free = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(lib().ptg_device_info(ctypes.byref(n), ctypes.byref(tot), ctypes.byref(free)), "ptg_device_info")
+    return {"num_cus": n.value, "total_mem": tot.value, "free_mem": free.value}
+
+
+class HostCsr:
+    """A CSR owned by the C library (V+1 offsets), exposed as numpy views."""
+
+    def __init__(self):
+        self._c = _HostCsr()
+        self._owned = False
+
+    def _adopt(self):
+        self._owned = True
+        V, E = self._c.V, self._c.E
+        self.offsets = np.ctypeslib.as_array(self._c.offsets, shape=(V + 1,))
+        self.cols = np.ctypeslib.as_array(self._c.cols, shape=(max(E, 1),))[:E]
+        self.weights = (np.ctypeslib.as_array(self._c.weights, shape=(max(E, 1),))[:E]
+                        if self._c.weights else None)
+        return self
+
+    @property
+    def V(self):
+        return self._c.V
+
+    @property
+    def E(self):
+        return self._c.E
+
+    @property
+    def node_pointers(self):
+        """The reference's V-entry pointer array (ptGraph/main.cu:257-264)."""
+        return self.offsets[: self.V]
+
+    def add_weights(self, lo=1, hi=64, seed=2, threads=0):
+        _check(lib().ptg_gen_weights(ctypes.byref(self._c), lo, hi, seed, threads), "ptg_gen_weights")
+        return self._adopt()
+
+    def free(self):
+        if self._owned:
+            self.offsets = self.cols = self.weights = None
+            lib().ptg_host_csr_free(ctypes.byref(self._c))
+            self._owned = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, threads=0):
+    h = HostCsr()
+    _check(lib().ptg_gen_rmat(scale, edge_factor, a, b, c, seed, threads, ctypes.byref(h._c)), "ptg_gen_rmat")
+    return h._adopt()
+
+
+def gen_er(scale, edge_factor=16, seed=1, threads=0):
+    h = HostCsr()
+    _check(lib().ptg_gen_er(scale, edge_factor, seed, threads, ctypes.byref(h._c)), "ptg_gen_er")
+    return h._adopt()
+
+
+def read_graph(path, fmt):
+    """fmt: 'bcsr', 'bwcsr' or 'java' (ptGraph/main.cu:98-116, :452-471, :2479-2573)."""
+    fn = {"bcsr": "ptg_read_bcsr", "bwcsr": "ptg_read_bwcsr", "java": "ptg_read_java_be"}[fmt]
+    h = HostCsr()
+    _check(getattr(lib(), fn)(os.fsencode(path), ctypes.byref(h._c)), fn)
+    return h._adopt()
+
+
+def write_graph(path, csr, fmt):
+    fn = {"bcsr": "ptg_write_bcsr", "bwcsr": "ptg_write_bwcsr"}[fmt]
+    c = _HostCsr()
+    offsets = np.ascontiguousarray(csr.offsets, dtype=np.uint32)
+    cols = np.ascontiguousarray(csr.cols, dtype=np.uint32)
+    w = None if csr.weights is None else np.ascontiguousarray(csr.weights, dtype=np.uint32)
+    c.V = offsets.size - 1
+    c.E = cols.size
+    c.offsets = _u32(offsets)
+    c.cols = _u32(cols) if cols.size else None
+    c.weights = _u32(w) if w is not None else None
+    _check(getattr(lib(), fn)(os.fsencode(path), ctypes.byref(c)), fn)
+
+
+def estimate_bytes(V, E, weighted, config=None):
+    cfg = config if config is not None else default_config()
+    return lib().ptg_graph_estimate_bytes(V, E, 1 if weighted else 0, ctypes.byref(cfg))
+
+
+def validate_csr(node_pointers, cols, E=None):
+    ptr = np.ascontiguousarray(node_pointers, dtype=np.uint32)
+    cols = np.ascontiguousarray(cols, dtype=np.uint32)
+    E = cols.size if E is None else E
+    return lib().ptg_validate_csr(ptr.size, E, _u32(ptr), _u32(cols) if cols.size else None, None)
+
+
+class DeviceBuffer:
+    """HBM buffer from ptg_device_alloc, for outputs that stay on the device."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        p = _vp()
+        _check(lib().ptg_device_alloc(nbytes, ctypes.byref(p)), "ptg_device_alloc")
+        self.ptr = p.value
+
+    def to_host(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        _check(lib().ptg_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "ptg_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().ptg_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A graph resident in HBM (ptg_graph_create).
+
+    node_pointers: V entries in the reference layout, or V+1 offsets when
+    `offsets_v1=True`; cols: uint32[E]; weights: uint32[E] or None.
+    """
+
+    def __init__(self, node_pointers, cols, weights=None, config=None, offsets_v1=False):
+        ptr = np.ascontiguousarray(node_pointers, dtype=np.uint32)
+        if offsets_v1:
+            ptr = np.ascontiguousarray(ptr[:-1])
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        self.V = ptr.size
+        self.E = cols.size
+        self.weighted = weights is not None
+        cfg = config if config is not None else default_config()
+        self.sentinel = cfg.sssp_sentinel
+        h = _vp()
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.uint32)
+        if w is not None and w.size == 0:
+            w = np.zeros(1, np.uint32)  # weighted graph without edges
+        rc = lib().ptg_graph_create_soa(self.V, self.E, _u32(ptr), _u32(cols) if cols.size else None,
+                                        _u32(w) if w is not None else None,
+                                        ctypes.byref(cfg), ctypes.byref(h))
+        _check(rc, "ptg_graph_create_soa")
+        self._h = h.value
+
+    @classmethod
+    def from_csr(cls, csr, weighted=True, config=None):
+        return cls(csr.node_pointers, csr.cols, csr.weights if weighted else None, config)
+
+    @classmethod
+    def from_aos(cls, node_pointers, edges, config=None):
+        """Weighted graph from an EdgeWithWeight array (ptg_graph_create)."""
+        ptr = np.ascontiguousarray(node_pointers, dtype=np.uint32)
+        edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+        self = cls.__new__(cls)
+        self.V, self.E, self.weighted = ptr.size, edges.shape[0], True
+        cfg = config if config is not None else default_config()
+        self.sentinel = cfg.sssp_sentinel
+        h = _vp()
+        rc = lib().ptg_graph_create(self.V, self.E, _u32(ptr), None,
+                                    edges.ctypes.data_as(ctypes.POINTER(EdgeWithWeight)),
+                                    ctypes.byref(cfg), ctypes.byref(h))
+        _check(rc, "ptg_graph_create")
+        self._h = h.value
+        return self
+
+    def _out(self, out, dtype):
+        if out is None:
+            return np.empty(self.V, dtype=dtype), None
+        if isinstance(out, DeviceBuffer):
+            return None, out.ptr
+        return out, None
+
+    def _run(self, fn, args, out, dtype, name):
+        st = Stats()
+        host, dev = self._out(out, dtype)
+        ptr = dev if dev is not None else host.ctypes.data
+        _check(fn(self._h, *args, ptr, ctypes.byref(st)), name)
+        return (host if dev is None else out), st
+
+    def bfs(self, src, out=None):
+        return self._run(lib().ptg_bfs, (src,), out, np.uint32, "ptg_bfs")
+
+    def sssp(self, src, out=None):
+        return self._run(lib().ptg_sssp, (src,), out, np.uint32, "ptg_sssp")
+
+    def cc(self, out=None):
+        return self._run(lib().ptg_cc, (), out, np.uint32, "ptg_cc")
+
+    def pagerank(self, damping=0.85, iters=20, out=None):
+        return self._run(lib().ptg_pagerank, (ctypes.c_float(damping), iters), out, np.float32, "ptg_pagerank")
+
+    def is_symmetric(self):
+        rc = lib().ptg_graph_is_symmetric(self._h)
+        if rc < 0:
+            raise PtgError(rc, "ptg_graph_is_symmetric")
+        return bool(rc)
+
+    @property
+    def device_bytes(self):
+        return lib().ptg_graph_device_bytes(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ptg_graph_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,244 @@
+"""GPU parity: the HIP path through libptgraph.so against the oracle and the
+golden fixtures. BFS levels, SSSP distances and CC labels are compared
+bit-exactly; PageRank (FP32) must be within 1e-6 L1 of the FP64 oracle."""
+import glob
+import hashlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import ptgraph as pg
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "hand_*.npz"))) + [
+    os.path.join(GOLDEN, "rmat8.npz"), os.path.join(GOLDEN, "rmat10.npz")]
+PR_L1 = 1e-6  # FP32 PageRank vs FP64 oracle, L1 over all vertices
+
+CONFIGS = {
+    "default": {},
+    "tiny_bins": {"small_degree": 1, "large_degree": 64},
+    "bottom_up": {"td_to_bu": 1e9, "bu_to_td": 1e9},
+    "top_down": {"td_to_bu": 1e-9},
+    "one_block": {"grid_blocks": 1},
+    "odd_grid": {"grid_blocks": 7, "small_degree": 3, "large_degree": 64},
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    pg.init(0)
+    if not os.path.exists(oracle._LIB_PATH):
+        oracle.build()
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def pr_l1(ptr, col, got, iters=20):
+    want = oracle.pagerank_f64(ptr, col, 0.85, iters)
+    return float(np.abs(got.astype(np.float64) - want).sum())
+
+
+def sort_rows(f):
+    # same graph with every adjacency list sorted (enables the symmetric-graph
+    # paths: bottom-up BFS and union-find CC)
+    ptr, col, w = f["ptr"], f["col"], f["w"]
+    E = col.size
+    nxt = np.append(ptr[1:], E)
+    col2, w2 = col.copy(), w.copy()
+    for u in range(ptr.size):
+        o = np.argsort(col[ptr[u]:nxt[u]], kind="stable")
+        col2[ptr[u]:nxt[u]] = col[ptr[u]:nxt[u]][o]
+        w2[ptr[u]:nxt[u]] = w[ptr[u]:nxt[u]][o]
+    g = dict(f)
+    g["col"], g["w"] = col2, w2
+    return g
+
+
+@pytest.mark.parametrize("rows", ["as_given", "sorted"])
+@pytest.mark.parametrize("cfg", list(CONFIGS), ids=list(CONFIGS))
+@pytest.mark.parametrize("path", FIXTURES, ids=os.path.basename)
+def test_fixture_parity(path, cfg, rows):
+    f = load(path)
+    if rows == "sorted":
+        f = sort_rows(f)
+    say("case %s cfg=%s rows=%s V=%d E=%d" % (os.path.basename(path), cfg, rows, f["ptr"].size, f["col"].size))
+    with pg.Graph(f["ptr"], f["col"], f["w"], config=pg.default_config(**CONFIGS[cfg])) as g:
+        for i, s in enumerate(f["srcs"]):
+            lv, st = g.bfs(int(s))
+            np.testing.assert_array_equal(lv, f["bfs"][i], err_msg="bfs src %d" % s)
+            reached = f["bfs"][i] != 0xFFFFFFFF
+            assert st.reached == int(reached.sum())
+            assert st.iterations == int(f["bfs"][i][reached].max())
+            d, _ = g.sssp(int(s))
+            np.testing.assert_array_equal(d, f["sssp"][i], err_msg="sssp src %d" % s)
+        lab, _ = g.cc()
+        np.testing.assert_array_equal(lab, f["cc"])
+        r, _ = g.pagerank(0.85, 20)
+        assert np.abs(r.astype(np.float64) - f["pr"]).sum() <= PR_L1
+
+
+@pytest.fixture(scope="module")
+def rmat16():
+    g = pg.gen_rmat(16, 16, seed=1).add_weights(1, 64, seed=2)
+    gg = pg.Graph.from_csr(g)
+    yield g, gg
+    gg.close()
+
+
+def test_rmat16_bfs_golden(rmat16):
+    g, gg = rmat16
+    say("case rmat16 bfs from 0")
+    meta = {}
+    with open(os.path.join(GOLDEN, "rmat16_bfs0.txt")) as fh:
+        for line in fh:
+            if not line.startswith("#"):
+                k, v = line.split()
+                meta[k] = v
+    lv, st = gg.bfs(0)
+    assert hashlib.sha256(lv.tobytes()).hexdigest() == meta["levels_sha256"]
+    assert st.iterations == int(meta["max_level"])
+    assert st.reached == int(meta["reached"])
+    assert st.edges_traversed == int(meta["edges_traversed"])
+
+
+def test_rmat16_vs_oracle(rmat16):
+    g, gg = rmat16
+    ptr, col, w = g.node_pointers, g.cols, g.weights
+    deg = np.diff(g.offsets.astype(np.int64))
+    rng = np.random.default_rng(3)
+    srcs = [0] + list(rng.choice(np.nonzero(deg > 0)[0], 6, replace=False)) + [int(np.nonzero(deg == 0)[0][0])]
+    for s in srcs:
+        say("case rmat16 src %d" % s)
+        lv, _ = gg.bfs(int(s))
+        want, _, _ = oracle.ref_loop(oracle.BFS, ptr, col, src=int(s), threads=8)
+        np.testing.assert_array_equal(lv, want)
+        d, st = gg.sssp(int(s))
+        np.testing.assert_array_equal(d, oracle.dijkstra(ptr, col, w, int(s)))
+    say("case rmat16 cc")
+    lab, st = gg.cc()
+    np.testing.assert_array_equal(lab, oracle.cc_reach_min(ptr, col))
+    assert st.reached == len(np.unique(lab))
+    say("case rmat16 pagerank")
+    for iters in (0, 1, 20):
+        r, _ = gg.pagerank(0.85, iters)
+        assert pr_l1(ptr, col, r, iters) <= PR_L1
+
+
+def test_rmat16_sentinel_opt_and_aos(rmat16):
+    g, _ = rmat16
+    say("case rmat16 aos + opt sentinel")
+    edges = np.stack([g.cols, g.weights], axis=1).astype(np.uint32)
+    cfg = pg.default_config(sssp_sentinel=pg.SSSP_SENTINEL_OPT)
+    with pg.Graph.from_aos(g.node_pointers, edges, config=cfg) as ga:
+        deg = np.diff(g.offsets.astype(np.int64))
+        s = int(np.nonzero(deg > 0)[0][100])
+        d, _ = ga.sssp(s)
+        np.testing.assert_array_equal(d, oracle.dijkstra(g.node_pointers, g.cols, g.weights, s,
+                                                         oracle.SSSP_SENTINEL_OPT))
+        assert (d == 0xFFFFFFFF).any()
+
+
+def test_device_outputs_and_determinism(rmat16):
+    g, gg = rmat16
+    say("case rmat16 device outputs")
+    buf = pg.DeviceBuffer(4 * g.V)
+    try:
+        h, _ = gg.bfs(5)
+        gg.bfs(5, out=buf)
+        np.testing.assert_array_equal(buf.to_host(np.uint32, g.V), h)
+        r1, _ = gg.pagerank(0.85, 20)
+        gg.pagerank(0.85, 20, out=buf)
+        np.testing.assert_array_equal(buf.to_host(np.float32, g.V), r1)
+    finally:
+        buf.free()
+    for _ in range(3):
+        r2, _ = gg.pagerank(0.85, 20)
+        np.testing.assert_array_equal(r2, r1)  # bitwise: fixed reduction order
+        l2, _ = gg.bfs(5)
+        np.testing.assert_array_equal(l2, h)
+        c2, _ = gg.cc()
+        np.testing.assert_array_equal(c2, oracle.cc_reach_min(g.node_pointers, g.cols))
+
+
+def random_directed(rng, V, E, wmax):
+    src = rng.integers(0, V, E)
+    dst = rng.integers(0, V, E)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    ptr = np.searchsorted(src, np.arange(V)).astype(np.uint32)
+    w = rng.integers(0, wmax + 1, E).astype(np.uint32)
+    return ptr, dst.astype(np.uint32), w
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_directed_graph(seed):
+    rng = np.random.default_rng(seed)
+    ptr, col, w = random_directed(rng, 3000, 9000, 9)
+    say("case directed seed %d" % seed)
+    with pg.Graph(ptr, col, w) as g:
+        assert not g.is_symmetric()
+        for s in (0, 17, 2999):
+            lv, _ = g.bfs(s)
+            np.testing.assert_array_equal(lv, oracle.bfs(ptr, col, s)[0])
+            d, _ = g.sssp(s)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ptr, col, w, s))
+        lab, _ = g.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_reach_min(ptr, col))
+        r, _ = g.pagerank(0.85, 20)
+        assert pr_l1(ptr, col, r) <= PR_L1
+
+
+def test_tiny_graphs():
+    say("case tiny graphs")
+    with pg.Graph(np.zeros(1, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32)) as g:
+        assert g.bfs(0)[0].tolist() == [1]
+        assert g.sssp(0)[0].tolist() == [1]
+        assert g.cc()[0].tolist() == [0]
+        r, _ = g.pagerank(0.85, 20)
+        assert abs(float(r[0]) - 1.0) < 1e-6
+    with pg.Graph(np.array([0, 1], np.uint32), np.array([1, 0], np.uint32), np.array([4, 4], np.uint32)) as g:
+        assert g.bfs(1)[0].tolist() == [2, 1]
+        assert g.sssp(0)[0].tolist() == [1, 5]
+        with pytest.raises(pg.PtgError) as e:
+            g.bfs(2)
+        assert e.value.code == pg.ERR_INVALID_ARG
+
+
+def test_unweighted_graph_rejects_sssp():
+    say("case unweighted sssp")
+    with pg.Graph(np.array([0, 1], np.uint32), np.array([1, 0], np.uint32)) as g:
+        with pytest.raises(pg.PtgError) as e:
+            g.sssp(0)
+        assert e.value.code == pg.ERR_UNSUPPORTED
+
+
+def test_reference_named_drivers(rmat16):
+    g, _ = rmat16
+    say("case reference-named drivers")
+    L = pg.lib()
+    ptr = np.ascontiguousarray(g.node_pointers)
+    col = np.ascontiguousarray(g.cols)
+    edges = np.ascontiguousarray(np.stack([g.cols, g.weights], axis=1).astype(np.uint32))
+    ep = edges.ctypes.data_as(__import__("ctypes").POINTER(pg.EdgeWithWeight))
+    for fn in ("bfsCaculateInShare", "bfsCaculateInOpt", "bfsCaculateInAsync", "bfsCaculateInAsyncSwap"):
+        assert getattr(L, fn)(g.V, g.E, pg._u32(ptr), pg._u32(col), 0) >= 0
+    assert L.ssspCaculateInShare(g.V, g.E, pg._u32(ptr), ep, 0) >= 0
+    assert L.ssspCaculateInOpt(g.V, g.E, pg._u32(ptr), ep, 0) >= 0
+    for fn in ("ccCaculateInShare", "ccCaculateInOpt", "prCaculateInShare", "prCaculateInOpt"):
+        assert getattr(L, fn)(g.V, g.E, pg._u32(ptr), pg._u32(col)) >= 0
+    bad = ptr.copy()
+    bad[3] = bad[4] + 1
+    assert L.bfsCaculateInShare(g.V, g.E, pg._u32(bad), pg._u32(col), 0) == pg.ERR_INVALID_GRAPH

@@ -6,10 +6,12 @@
 //   r0(v) = 1/V
 //   r'(v) = (1-d)/V + d * ( sum_{u->v} r(u)/outdeg(u) + D/V ),
 //   D     = sum of r(u) over vertices with outdeg(u) = 0.
-// One kernel per iteration over static in-degree bins (hub lists split into
-// chunks whose partials are combined in chunk order by the last chunk to
-// finish). Every sum is a (hi, lo) pair updated with Knuth's TwoSum, and every
-// reduction runs in a fixed order, so results are bit-identical run to run.
+// Per iteration, a gather kernel over static in-degree bins (hub lists split
+// into chunks) and a one-workgroup kernel that combines the hub chunks and
+// the dangling mass; the kernel boundary orders them, so no inter-workgroup
+// fences or counters are needed. Every sum is a (hi, lo) pair updated with
+// Knuth's TwoSum, and every reduction runs in a fixed order, so results are
+// bit-identical run to run.
 #include <string.h>
 
 #include <algorithm>
@@ -59,10 +61,6 @@ __device__ __forceinline__ CF block_cf_sum(CF a, CF *lds) {
     return t;
 }
 
-__device__ __forceinline__ float load_coherent(const float *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 struct PrArgs {
     Frontier bins;                  // in-degree bins
     const uint32_t *zero_list;      // vertices with in-degree 0
@@ -74,26 +72,39 @@ struct PrArgs {
     float *rank_out;                // ranks (last iteration only) or null
     const float *dang_in;           // (hi, lo) of D for the previous iteration
     float *dang_out;
-    float *part;                    // per-workgroup (hi, lo)
-    uint32_t *sync;                 // workgroup completion counter
+    float *part;                    // per-workgroup dangling partials (hi, lo)
     float *lpart;                   // hub chunk partials (hi, lo)
-    uint32_t *lcnt;                 // hub chunk completion counters
+    uint32_t gather_blocks;         // workgroups of the gather kernel
     float damping, inv_v;
 };
 
-__global__ __launch_bounds__(kBlock) void k_pr_iter(PrArgs a) {
-    __shared__ CF lds[kWavesPerBlock];
-    __shared__ uint32_t is_last;
-    const float dang = a.dang_in[0] + a.dang_in[1];
-    const float base = (1.0f - a.damping) * a.inv_v + a.damping * (dang * a.inv_v);
-    CF my_dang{0.0f, 0.0f};
-    auto finalize = [&](uint32_t v, float sum) {
+struct PrVertex {
+    const PrArgs &a;
+    float base;
+    CF &dang;
+    // r = base + d * sum; contrib = r / outdeg (0 for a dangling vertex,
+    // whose r joins the dangling mass instead).
+    __device__ __forceinline__ void finalize(uint32_t v, float sum) const {
         const float r = base + a.damping * sum;
         const uint32_t od = a.ooff[v + 1] - a.ooff[v];
         a.cout[v] = od ? r / (float)od : 0.0f;
         if (a.rank_out) a.rank_out[v] = r;
-        if (!od) cf_add(my_dang, r);
-    };
+        if (!od) cf_add(dang, r);
+    }
+};
+
+__device__ __forceinline__ float pr_base(const PrArgs &a) {
+    const float dang = a.dang_in[0] + a.dang_in[1];
+    return (1.0f - a.damping) * a.inv_v + a.damping * (dang * a.inv_v);
+}
+
+// Gathers every vertex's in-neighbour contributions. Hub lists are cut into
+// chunks whose partial sums go to lpart; k_pr_combine finishes those
+// vertices. Each workgroup leaves its share of the dangling mass in part.
+__global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
+    __shared__ CF lds[kWavesPerBlock];
+    CF my_dang{0.0f, 0.0f};
+    const PrVertex pv{a, pr_base(a), my_dang};
     const uint32_t lane = lane_id();
 
     // Hubs: one workgroup per chunk of kChunkEdges in-edges.
@@ -107,27 +118,14 @@ __global__ __launch_bounds__(kBlock) void k_pr_iter(PrArgs a) {
             if (a.bins.lb[mid] <= c) lo = mid; else hi = mid;
         }
         const uint32_t v = a.bins.ql[lo];
-        const uint32_t first = a.bins.lb[lo];
-        const uint64_t vb = a.ioff[v], ve = a.ioff[v + 1];
-        const uint64_t b = vb + (uint64_t)(c - first) * kChunkEdges;
-        const uint64_t e = min(ve, b + kChunkEdges);
+        const uint64_t b = (uint64_t)a.ioff[v] + (uint64_t)(c - a.bins.lb[lo]) * kChunkEdges;
+        const uint64_t e = min((uint64_t)a.ioff[v + 1], b + kChunkEdges);
         CF acc{0.0f, 0.0f};
         for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) cf_add(acc, a.cin[a.icol[i]]);
         acc = block_cf_sum(acc, lds);
         if (threadIdx.x == 0) {
             a.lpart[2ull * c] = acc.hi;
             a.lpart[2ull * c + 1] = acc.lo;
-            __threadfence();
-            const uint32_t nchv = (uint32_t)((ve - vb + kChunkEdges - 1) / kChunkEdges);
-            if (atomicAdd(&a.lcnt[lo], 1u) == nchv - 1) {
-                __threadfence();
-                CF t{0.0f, 0.0f};
-                for (uint32_t j = 0; j < nchv; j++)
-                    t = cf_merge(t, CF{load_coherent(&a.lpart[2ull * (first + j)]),
-                                       load_coherent(&a.lpart[2ull * (first + j) + 1])});
-                finalize(v, t.hi + t.lo);
-                a.lcnt[lo] = 0u;
-            }
         }
     }
 
@@ -141,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_iter(PrArgs a) {
         CF acc{0.0f, 0.0f};
         for (uint64_t k = b + lane; k < e; k += 64) cf_add(acc, a.cin[a.icol[k]]);
         acc = wave_cf_sum(acc);
-        if (lane == 0) finalize(v, acc.hi + acc.lo);
+        if (lane == 0) pv.finalize(v, acc.hi + acc.lo);
     }
 
     // Small in-degree: one lane per vertex; then in-degree 0.
@@ -153,31 +151,39 @@ __global__ __launch_bounds__(kBlock) void k_pr_iter(PrArgs a) {
         const uint32_t b = a.ioff[v], e = a.ioff[v + 1];
         CF acc{0.0f, 0.0f};
         for (uint32_t k = b; k < e; k++) cf_add(acc, a.cin[a.icol[k]]);
-        finalize(v, acc.hi + acc.lo);
+        pv.finalize(v, acc.hi + acc.lo);
     }
-    for (uint32_t i = gt; i < a.n_zero; i += nt) finalize(a.zero_list[i], 0.0f);
+    for (uint32_t i = gt; i < a.n_zero; i += nt) pv.finalize(a.zero_list[i], 0.0f);
 
-    // Dangling mass: per-workgroup partials, summed in index order by the
-    // last workgroup to finish.
     const CF bd = block_cf_sum(my_dang, lds);
     if (threadIdx.x == 0) {
         a.part[2 * blockIdx.x] = bd.hi;
         a.part[2 * blockIdx.x + 1] = bd.lo;
-        __threadfence();
-        is_last = atomicAdd(&a.sync[0], 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
+}
+
+// One workgroup: finishes the hub vertices from their chunk partials (in chunk
+// order) and sums the dangling mass of all gather workgroups (in index order).
+__global__ __launch_bounds__(kBlock) void k_pr_combine(PrArgs a) {
+    __shared__ CF lds[kWavesPerBlock];
+    CF my_dang{0.0f, 0.0f};
+    const PrVertex pv{a, pr_base(a), my_dang};
+    const unsigned long long lp = a.bins.ctr->l_pack;
+    const uint32_t nl = (uint32_t)(lp >> kLCountShift);
+    for (uint32_t l = threadIdx.x; l < nl; l += kBlock) {
+        const uint32_t v = a.bins.ql[l];
+        const uint32_t first = a.bins.lb[l];
+        const uint32_t n = bin_chunks(a.ioff[v + 1] - a.ioff[v]);
         CF t{0.0f, 0.0f};
-        for (uint32_t j = threadIdx.x; j < gridDim.x; j += kBlock)
-            t = cf_merge(t, CF{load_coherent(&a.part[2 * j]), load_coherent(&a.part[2 * j + 1])});
-        t = block_cf_sum(t, lds);
-        if (threadIdx.x == 0) {
-            a.dang_out[0] = t.hi;
-            a.dang_out[1] = t.lo;
-            a.sync[0] = 0u;
-        }
+        for (uint32_t j = 0; j < n; j++) t = cf_merge(t, CF{a.lpart[2ull * (first + j)], a.lpart[2ull * (first + j) + 1]});
+        pv.finalize(v, t.hi + t.lo);
+    }
+    for (uint32_t j = threadIdx.x; j < a.gather_blocks; j += kBlock)
+        my_dang = cf_merge(my_dang, CF{a.part[2 * j], a.part[2 * j + 1]});
+    const CF t = block_cf_sum(my_dang, lds);
+    if (threadIdx.x == 0) {
+        a.dang_out[0] = t.hi;
+        a.dang_out[1] = t.lo;
     }
 }
 
@@ -236,18 +242,13 @@ static int ensure_pr_state(ptg_graph *g) {
     if (g->d_contrib[0]) return PTG_OK;
     int rc;
     const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
-    const uint64_t lents = g->in.bins.h_ctr.l_pack >> kLCountShift;
     if ((rc = galloc(g, (void **)&g->d_contrib[0], sizeof(float) * g->V)) ||
         (rc = galloc(g, (void **)&g->d_contrib[1], sizeof(float) * g->V)) ||
         (rc = galloc(g, (void **)&g->d_dang, sizeof(float) * 4)) ||
         (rc = galloc(g, (void **)&g->d_part, sizeof(float) * 2 * g->grid)) ||
-        (rc = galloc(g, (void **)&g->d_sync, sizeof(uint32_t) * 4)) ||
-        (rc = galloc(g, (void **)&g->d_lpart, sizeof(float) * 2 * (chunks + 1))) ||
-        (rc = galloc(g, (void **)&g->d_lcnt, sizeof(uint32_t) * (lents + 1))))
+        (rc = galloc(g, (void **)&g->d_lpart, sizeof(float) * 2 * (chunks + 1))))
         return rc;
     g->lchunks_cap = (uint32_t)chunks;
-    g->lcnt_cap = (uint32_t)lents;
-    if ((rc = fill_u32(g, g->d_sync, 4, 0u)) || (rc = fill_u32(g, g->d_lcnt, lents + 1, 0u))) return rc;
     return PTG_OK;
 }
 
@@ -287,14 +288,14 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
             a.dang_in = g->d_dang + 2 * (t & 1);
             a.dang_out = g->d_dang + 2 * ((t + 1) & 1);
             a.part = g->d_part;
-            a.sync = g->d_sync;
             a.lpart = g->d_lpart;
-            a.lcnt = g->d_lcnt;
+            a.gather_blocks = g->grid;
             a.damping = damping;
             a.inv_v = r0;
             timer.before();
-            k_pr_iter<<<g->grid, kBlock, 0, g->stream>>>(a);
+            k_pr_gather<<<g->grid, kBlock, 0, g->stream>>>(a);
             timer.after();
+            k_pr_combine<<<1, kBlock, 0, g->stream>>>(a);
             PTG_HIP(hipGetLastError());
         }
     }

@@ -26,6 +26,33 @@ constexpr int kBlock = 256;  // threads per workgroup (4 waves of 64)
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr uint32_t kChunkEdges = 2048;  // edges per workgroup task for hub vertices
 constexpr uint32_t kRing = 4096;        // per-round counter blocks kept on the device
+constexpr int kShards = 16;             // counters and queues are split 16 ways
+constexpr uint32_t kQueueCap = 16384;   // per-shard queue capacity = max queued frontier
+constexpr uint32_t kModeQueue = 0, kModeBitmap = 1;
+constexpr uint32_t kDirTopDown = 0, kDirBottomUp = 1;
+
+constexpr uint32_t kWaveChunk = 256;   // edges per chunk of the frontier chunk queue
+
+// Description of the frontier consumed by round r (ring[r % kRing]) and the
+// state carried into that round. Written by round r - 1 (and, for bitmap
+// frontiers, the chunk queue by the pre-pass of round r): header fields by one
+// thread, counts through sharded atomics (workgroup b uses shard b % 16).
+// A frontier vertex of degree d is queued in the S queue (0 < d <= sd) or the
+// chunk queue (d > sd, ceil(d / kWaveChunk) chunks).
+struct RoundCtr {
+    uint32_t qn[kShards];                   // S-queue entries per shard
+    unsigned long long cpack[kShards];      // chunk queue per shard: (entries << 40) | chunks
+    uint32_t mode;                          // kModeQueue or kModeBitmap
+    uint32_t dir;                           // BFS: direction of the producing round
+    uint32_t thr;                           // SSSP: near/far threshold of this frontier
+    uint32_t far_base;                      // SSSP: far-set size before this round's adds
+    unsigned long long seen;                // BFS: sum of out-degrees of earlier frontiers
+    uint32_t pad[10];
+    uint32_t found_sh[kShards];             // frontier vertices (any degree)
+    uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
+    unsigned long long edges_sh[kShards];   // sum of out-degrees of the frontier
+};
+static_assert(sizeof(RoundCtr) == 512, "RoundCtr layout");
 
 // Counters of one frontier. Bins: S = lane per vertex, M = wave per vertex,
 // L = edge chunks of kChunkEdges spread over workgroups.
@@ -99,23 +126,25 @@ struct ptg_graph {
 
     // Query state, allocated once per graph and reused by every query.
     uint32_t *d_val = nullptr;          // V: levels / distances / labels
-    uint32_t *d_bits[3] = {};           // bitmaps of nwords words
+    uint32_t *d_bits[5] = {};           // bitmaps of nwords words: 3 rotating frontiers,
+                                        // visited (BFS) / far set (SSSP), scratch flags
     uint32_t nwords = 0;                // 2 * npairs
     uint32_t npairs = 0;                // ceil(V / 64)
-    ptg::QueueSet q[2] = {};            // double-buffered frontier queues
+    uint32_t *d_queue[2] = {};          // sharded S queues (kShards x kQueueCap)
+    uint32_t *d_cq[2] = {};             // sharded chunk queues: vertex ids (kShards x cq_cap)
+    uint32_t *d_cb[2] = {};             // sharded chunk queues: first-chunk indices
+    uint32_t cq_cap = 0;                // chunk-queue entries per shard
     uint32_t *d_qmem = nullptr;
-    ptg::BinCounters *d_ring = nullptr; // kRing per-round counter blocks
-    ptg::BinCounters *h_ring = nullptr; // pinned mirror
+    ptg::RoundCtr *d_ring = nullptr;    // kRing per-round counter blocks
+    ptg::RoundCtr *h_ring = nullptr;    // pinned mirror
     unsigned long long *d_acc = nullptr; // 4 accumulators for statistics kernels
 
     // PageRank state.
     float *d_contrib[2] = {};
     float *d_dang = nullptr;            // 2 x (hi, lo)
-    float *d_part = nullptr;            // per-workgroup (hi, lo)
-    uint32_t *d_sync = nullptr;         // [0] block completion counter
+    float *d_part = nullptr;            // per-workgroup dangling partials (hi, lo)
     float *d_lpart = nullptr;           // hub chunk partials (hi, lo)
-    uint32_t *d_lcnt = nullptr;         // hub chunk completion counters
-    uint32_t lchunks_cap = 0, lcnt_cap = 0;
+    uint32_t lchunks_cap = 0;
 
     // Per-launch timing of the main kernel (PTG_FLAG_TIMING).
     std::vector<hipEvent_t> tev;  // pairs: [2i] before, [2i+1] after launch i
@@ -132,6 +161,9 @@ namespace ptg {
 int galloc(ptg_graph *g, void **p, uint64_t bytes);
 void gfree(ptg_graph *g, void *p);
 bool is_device_pointer(const void *p);
+
+// Entries per shard of a chunk queue for V vertices and `grid` workgroups.
+uint32_t chunk_queue_cap(uint32_t V, uint32_t grid);
 
 // Host-built degree bins uploaded to the device.
 int build_static_bins(ptg_graph *g, const uint32_t *h_off, uint32_t V, StaticBins *sb);

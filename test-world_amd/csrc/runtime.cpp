@@ -55,6 +55,17 @@ void gfree(ptg_graph *, void *p) {
     if (p) (void)hipFree(p);
 }
 
+uint32_t chunk_queue_cap(uint32_t V, uint32_t grid) {
+    // The bitmap pre-pass gives workgroup b the tiles [b*T, (b+1)*T) and
+    // appends to shard b % kShards; queue-mode rounds append at most
+    // min(V, kQueueCap) entries to any shard.
+    const uint64_t npairs = (V + 63) / 64;
+    const uint64_t T = (npairs + grid - 1) / grid;
+    const uint64_t blocks_per_shard = (grid + kShards - 1) / kShards;
+    const uint64_t cap = std::max<uint64_t>(blocks_per_shard * 64 * T, std::min<uint64_t>(V, kQueueCap));
+    return (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
+}
+
 int build_static_bins(ptg_graph *g, const uint32_t *h_off, uint32_t V, StaticBins *sb) {
     const uint32_t sd = g->cfg.small_degree, ld = g->cfg.large_degree;
     std::vector<uint32_t> s, m, l, lb, z;
@@ -172,9 +183,7 @@ static void destroy(ptg_graph *g) {
     for (auto *c : g->d_contrib) gfree(g, c);
     gfree(g, g->d_dang);
     gfree(g, g->d_part);
-    gfree(g, g->d_sync);
     gfree(g, g->d_lpart);
-    gfree(g, g->d_lcnt);
     if (g->h_ring) (void)hipHostFree(g->h_ring);
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
@@ -272,19 +281,21 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     g->nwords = 2 * g->npairs;
     TRY(galloc(g, (void **)&g->d_val, sizeof(uint32_t) * V));
     for (auto &b : g->d_bits) TRY(galloc(g, (void **)&b, sizeof(uint32_t) * g->nwords));
-    const uint64_t lcap = std::min<uint64_t>(V, E / (c.large_degree + 1) + 1);
-    const uint64_t per_set = 2ull * V + 2 * lcap + 8;
+    // per buffer: sharded S queue and sharded chunk queue (ids + chunk bases)
+    g->cq_cap = chunk_queue_cap(V, g->grid);
+    const uint64_t qwords = (uint64_t)kShards * kQueueCap;
+    const uint64_t cwords = (uint64_t)kShards * g->cq_cap;
+    const uint64_t per_set = qwords + 2 * cwords;
     TRY(galloc(g, (void **)&g->d_qmem, sizeof(uint32_t) * 2 * per_set));
     for (int i = 0; i < 2; i++) {
         uint32_t *p = g->d_qmem + i * per_set;
-        g->q[i].qs = p;
-        g->q[i].qm = p + V;
-        g->q[i].ql = p + 2ull * V;
-        g->q[i].lb = p + 2ull * V + lcap;
+        g->d_queue[i] = p;
+        g->d_cq[i] = p + qwords;
+        g->d_cb[i] = p + qwords + cwords;
     }
-    TRY(galloc(g, (void **)&g->d_ring, sizeof(BinCounters) * kRing));
+    TRY(galloc(g, (void **)&g->d_ring, sizeof(RoundCtr) * kRing));
     TRY(galloc(g, (void **)&g->d_acc, sizeof(unsigned long long) * 4));
-    if (hipHostMalloc((void **)&g->h_ring, sizeof(BinCounters) * kRing, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&g->h_ring, sizeof(RoundCtr) * kRing, hipHostMallocDefault) != hipSuccess)
         TRY(fail(PTG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters)"));
 #undef TRY
     *out = g;
@@ -420,12 +431,13 @@ uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const pt
     const uint64_t large = c.large_degree ? c.large_degree : 4096;
     const uint64_t lcap = std::min<uint64_t>(V, E / (large + 1) + 1);
     const uint64_t nwords = 2ull * ((V + 63) / 64);
+    const uint32_t grid = c.grid_blocks ? c.grid_blocks : 2048u;
     auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
     uint64_t b = al(4 * v1) + al(4 * E) + (weighted ? al(4 * E) : 0);  // CSR
     b += al(4 * (uint64_t)V + 4 * (2 * lcap) + 64);                    // static bins
-    b += al(4 * (uint64_t)V) + 3 * al(4 * nwords);                     // values + bitmaps
-    b += al(4 * 2 * (2ull * V + 2 * lcap + 8));                        // frontier queues
-    b += al(sizeof(BinCounters) * kRing) + al(32);                     // counter ring, accumulators
+    b += al(4 * (uint64_t)V) + 5 * al(4 * nwords);                     // values + bitmaps
+    b += al(4 * 2 * ((uint64_t)kShards * kQueueCap + 2ull * kShards * chunk_queue_cap(V, grid)));  // queues
+    b += al(sizeof(RoundCtr) * kRing) + al(32);                        // counter ring, accumulators
     return b;
 }
 

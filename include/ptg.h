@@ -63,6 +63,7 @@ enum {
 /* Graph-creation flags (ptg_config.flags). */
 #define PTG_FLAG_NO_VALIDATE 0x1u  /* skip the host-side CSR validation */
 #define PTG_FLAG_TIMING 0x2u       /* time every main-kernel launch with HIP events */
+#define PTG_FLAG_NO_RELABEL 0x4u   /* keep the caller's vertex order on the device */
 
 typedef struct ptg_config {
     uint64_t device_budget_bytes; /* 0 = whole device; cap on graph + state */

@@ -1,0 +1,165 @@
+// Connected components for gfx950.
+//
+// The reference's cc_kernel (ptGraph/gpu_kernels.cu:136-150) propagates
+// value[v] = v along out-edges to a fixed point: label[v] = the smallest id u
+// with a path u -> v. On a symmetric graph that is the smallest id of v's
+// component, which union-find with min-id roots computes in one pass over the
+// edges; on other graphs the frontier label propagation (sssp.hip) runs.
+// Ids are the caller's: with relabeling the union-find keys are the original
+// ids, so every root is its component's minimum original id.
+#include <string.h>
+
+#include "rounds.h"
+#include "static_expand.h"
+
+namespace ptg {
+
+int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uint32_t *iters, uint64_t *fsum);
+
+// Root search with path halving. A parent always has a smaller key than its
+// child, so any value read, even a stale one, is an ancestor and the walk
+// terminates.
+__device__ __forceinline__ uint32_t uf_root(uint32_t *par, uint32_t x) {
+    for (;;) {
+        const uint32_t p = par[x];
+        if (p == x) return x;
+        const uint32_t gp = par[p];
+        if (gp == p) return p;
+        par[x] = gp;
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ uint32_t uf_key(const uint32_t *key, uint32_t x) { return key ? key[x] : x; }
+
+// Links the two trees, always hanging the root with the larger key under the
+// other one, so every root is the minimum-key vertex of its tree.
+__device__ __forceinline__ void uf_unite(uint32_t *par, const uint32_t *key, uint32_t a, uint32_t b) {
+    uint32_t ra = uf_root(par, a), rb = uf_root(par, b);
+    while (ra != rb) {
+        if (uf_key(key, ra) < uf_key(key, rb)) {
+            const uint32_t t = ra;
+            ra = rb;
+            rb = t;
+        }
+        const uint32_t seen = atomicCAS(&par[ra], ra, rb);
+        if (seen == ra) return;
+        ra = uf_root(par, seen);
+        rb = uf_root(par, rb);
+    }
+}
+
+struct Hook {
+    const uint32_t *__restrict__ col;
+    const uint32_t *__restrict__ key;
+    uint32_t *par;
+    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return v; }
+    __device__ __forceinline__ void edge(bool valid, uint32_t v, uint64_t e) const {
+        if (!valid) return;
+        const uint32_t u = col[e];
+        if (u < v) uf_unite(par, key, v, u);
+    }
+};
+
+// par[v] = v, or its first neighbour when that one has a smaller key (a
+// valid first hook).
+__global__ __launch_bounds__(kBlock) void k_uf_init(uint32_t V, const uint32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ col,
+                                                    const uint32_t *__restrict__ key, uint32_t *__restrict__ par) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        const uint32_t b = off[v];
+        uint32_t p = v;
+        if (off[v + 1] > b && uf_key(key, col[b]) < uf_key(key, v)) p = col[b];
+        par[v] = p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_uf_flatten(uint32_t V, uint32_t *par) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) par[v] = uf_root(par, v);
+}
+
+// label[v] = key of its root (after flattening, par[v] is the root).
+__global__ __launch_bounds__(kBlock) void k_map_keys(uint32_t V, const uint32_t *__restrict__ key,
+                                                     uint32_t *__restrict__ par) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) par[v] = key[par[v]];
+}
+
+// value[v] = key of v (label propagation starts from the caller's ids).
+__global__ __launch_bounds__(kBlock) void k_init_keys(uint32_t V, const uint32_t *__restrict__ key,
+                                                      uint32_t *__restrict__ val) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) val[v] = key ? key[v] : v;
+}
+
+// Number of v whose label is its own key, one per component (added to 1 of
+// 4 accumulators).
+__global__ __launch_bounds__(kBlock) void k_count_roots(uint32_t V, const uint32_t *__restrict__ label,
+                                                        const uint32_t *__restrict__ key, unsigned long long *out) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    uint64_t cnt = 0;
+    for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64) {
+        const uint64_t v = base + lane;
+        cnt += (v < V && label[v] == (key ? key[v] : (uint32_t)v));
+    }
+    cnt = wave_sum_u64(cnt);
+    if (lane == 0 && cnt) atomicAdd(&out[blockIdx.x % 4], (unsigned long long)cnt);
+}
+
+static int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots) {
+    unsigned long long *acc = g->d_acc;
+    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 8, 0u);
+    if (rc) return rc;
+    k_count_roots<<<g->grid, kBlock, 0, g->stream>>>(g->V, label, g->d_inv, acc);
+    PTG_HIP(hipGetLastError());
+    unsigned long long h[4] = {0, 0, 0, 0};
+    PTG_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    *roots = (uint32_t)(h[0] + h[1] + h[2] + h[3]);
+    return PTG_OK;
+}
+
+int run_cc(ptg_graph *g, uint32_t *label, ptg_stats *st) {
+    int sym = 0;
+    int rc = check_symmetric(g, &sym);
+    if (rc) return rc;
+    uint32_t iters = 1;
+    uint64_t fsum = g->V;
+    LaunchTimer timer(g);
+    const uint32_t *key = g->d_inv;  // null when the device keeps the caller's ids
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if (sym) {
+        k_uf_init<<<g->grid, kBlock, 0, g->stream>>>(g->V, g->out.off, g->out.col, key, label);
+        const Frontier fin = g->out.bins.q.with(g->out.bins.d_ctr);
+        timer.launch(k_expand<Hook>, dim3(g->grid), dim3(kBlock), fin, (const uint32_t *)g->out.off,
+                     Hook{g->out.col, key, label});
+        k_uf_flatten<<<g->grid, kBlock, 0, g->stream>>>(g->V, label);
+        if (key) k_map_keys<<<g->grid, kBlock, 0, g->stream>>>(g->V, key, label);
+        PTG_HIP(hipGetLastError());
+    } else {
+        k_init_keys<<<g->grid, kBlock, 0, g->stream>>>(g->V, key, label);
+        PTG_HIP(hipGetLastError());
+        if ((rc = run_label_propagation(g, label, timer, &iters, &fsum))) return rc;
+    }
+    PTG_HIP(hipEventRecord(g->ev1, g->stream));
+    PTG_HIP(hipEventSynchronize(g->ev1));
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        st->kernel_ns = (uint64_t)((double)ms * 1e6);
+        st->iterations = iters;
+        st->frontier_sum = fsum;
+        st->edges_traversed = g->E;
+        if ((rc = timer.collect(st))) return rc;
+        // reached = number of distinct labels (components on a symmetric graph)
+        if ((rc = count_roots(g, label, &st->reached))) return rc;
+    }
+    return PTG_OK;
+}
+
+}  // namespace ptg

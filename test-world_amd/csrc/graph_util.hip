@@ -1,0 +1,140 @@
+// Small device utilities: fills, the result un-permutation, the one-time
+// symmetry check and reached-set statistics.
+#include <string.h>
+
+#include <algorithm>
+
+#include "static_expand.h"
+
+namespace ptg {
+
+// Fills n words; `vec` (16-byte aligned p only) writes uint4 per lane.
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v, bool vec) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n4 = vec ? n / 4 : 0;
+    uint4 *p4 = reinterpret_cast<uint4 *>(p);
+    const uint4 vv = make_uint4(v, v, v, v);
+    for (uint64_t j = i; j < n4; j += stride) p4[j] = vv;
+    for (uint64_t j = n4 * 4 + i; j < n; j += stride) p[j] = v;
+}
+
+int fill_u32(ptg_graph *g, uint32_t *p, uint64_t n, uint32_t v) {
+    if (n == 0) return PTG_OK;
+    const bool vec = ((uintptr_t)p & 15u) == 0;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(((vec ? n / 4 : n) + kBlock - 1) / kBlock + 1, g->grid);
+    k_fill_u32<<<blocks, kBlock, 0, g->stream>>>(p, n, v, vec);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
+// out[v] = src[perm[v]]: a result in device ids back to the caller's ids.
+__global__ __launch_bounds__(kBlock) void k_unpermute(uint32_t V, const uint32_t *__restrict__ perm,
+                                                      const uint32_t *__restrict__ src, uint32_t *__restrict__ dst) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) dst[v] = src[perm[v]];
+}
+
+int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst) {
+    k_unpermute<<<g->grid, kBlock, 0, g->stream>>>(g->V, g->d_perm, src, dst);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetry check (once per graph, at creation): every adjacency sorted
+// ascending and every edge (v, u) matched by (u, v), found by binary search.
+// ---------------------------------------------------------------------------
+
+struct CheckSorted {
+    const uint32_t *__restrict__ off;
+    const uint32_t *__restrict__ col;
+    uint32_t *flag;
+    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return off[v + 1]; }
+    __device__ __forceinline__ void edge(bool valid, uint32_t end, uint64_t e) const {
+        if (valid && e + 1 < end && col[e] > col[e + 1]) *flag = 0u;
+    }
+};
+
+struct CheckReverse {
+    const uint32_t *__restrict__ off;
+    const uint32_t *__restrict__ col;
+    uint32_t *flag;
+    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return v; }
+    __device__ __forceinline__ void edge(bool valid, uint32_t v, uint64_t e) const {
+        if (!valid) return;
+        const uint32_t u = col[e];
+        uint32_t lo = off[u], hi = off[u + 1];
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (col[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        if (lo == off[u + 1] || col[lo] != v) *flag = 0u;
+    }
+};
+
+int check_symmetric(ptg_graph *g, int *sym) {
+    if (g->symmetric >= 0) {
+        *sym = g->symmetric;
+        return PTG_OK;
+    }
+    uint32_t *flag = g->d_bits[4];
+    int rc = fill_u32(g, flag, 2, 1u);
+    if (rc) return rc;
+    const Frontier fin = g->out.bins.q.with(g->out.bins.d_ctr);
+    k_expand<CheckSorted><<<g->grid, kBlock, 0, g->stream>>>(fin, g->out.off, CheckSorted{g->out.off, g->out.col, flag});
+    k_expand<CheckReverse><<<g->grid, kBlock, 0, g->stream>>>(fin, g->out.off,
+                                                              CheckReverse{g->out.off, g->out.col, flag + 1});
+    PTG_HIP(hipGetLastError());
+    uint32_t h[2];
+    PTG_HIP(hipMemcpyAsync(h, flag, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    // The reverse search assumes sorted lists; unsorted means "not proven".
+    g->symmetric = (h[0] && h[1]) ? 1 : 0;
+    *sym = g->symmetric;
+    return PTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reached count and m_r = sum of out-degrees of reached vertices.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32_t *__restrict__ val,
+                                                        uint32_t unreached, const uint32_t *__restrict__ off,
+                                                        unsigned long long *out) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    uint64_t cnt = 0, edges = 0;
+    for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64) {
+        const uint64_t v = base + lane;
+        if (v < V && val[v] != unreached) {
+            cnt++;
+            edges += off[v + 1] - off[v];
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    edges = wave_sum_u64(edges);
+    if (lane == 0 && cnt) {
+        const uint32_t s = 2 * (blockIdx.x % 2);
+        atomicAdd(&out[s], (unsigned long long)cnt);
+        atomicAdd(&out[s + 1], (unsigned long long)edges);
+    }
+}
+
+int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached, uint64_t *edges) {
+    if (!reached && !edges) return PTG_OK;
+    unsigned long long *acc = g->d_acc;
+    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 8, 0u);
+    if (rc) return rc;
+    k_reach_stats<<<g->grid, kBlock, 0, g->stream>>>(g->V, val, unreached, g->out.off, acc);
+    PTG_HIP(hipGetLastError());
+    unsigned long long h[4];
+    PTG_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    if (reached) *reached = (uint32_t)(h[0] + h[2]);
+    if (edges) *edges = h[1] + h[3];
+    return PTG_OK;
+}
+
+}  // namespace ptg

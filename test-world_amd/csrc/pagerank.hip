@@ -292,9 +292,7 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
             a.gather_blocks = g->grid;
             a.damping = damping;
             a.inv_v = r0;
-            timer.before();
-            k_pr_gather<<<g->grid, kBlock, 0, g->stream>>>(a);
-            timer.after();
+            timer.launch(k_pr_gather, dim3(g->grid), dim3(kBlock), a);
             k_pr_combine<<<1, kBlock, 0, g->stream>>>(a);
             PTG_HIP(hipGetLastError());
         }

@@ -1,6 +1,7 @@
 // Internal declarations shared by the host runtime and the HIP kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -42,12 +43,13 @@ constexpr uint32_t kWaveChunk = 256;   // edges per chunk of the frontier chunk 
 struct RoundCtr {
     uint32_t qn[kShards];                   // S-queue entries per shard
     unsigned long long cpack[kShards];      // chunk queue per shard: (entries << 40) | chunks
-    uint32_t mode;                          // kModeQueue or kModeBitmap
+    uint32_t mode;                          // representation (queue, bitmap or flags)
     uint32_t dir;                           // BFS: direction of the producing round
     uint32_t thr;                           // SSSP: near/far threshold of this frontier
-    uint32_t far_base;                      // SSSP: far-set size before this round's adds
+    uint32_t any;                           // 1 if the producing round emitted anything
     unsigned long long seen;                // BFS: sum of out-degrees of earlier frontiers
-    uint32_t pad[10];
+    uint32_t far_any;                       // SSSP: far set non-empty after the producing round
+    uint32_t pad[9];
     uint32_t found_sh[kShards];             // frontier vertices (any degree)
     uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
     unsigned long long edges_sh[kShards];   // sum of out-degrees of the frontier
@@ -127,7 +129,9 @@ struct ptg_graph {
     // Query state, allocated once per graph and reused by every query.
     uint32_t *d_val = nullptr;          // V: levels / distances / labels
     uint32_t *d_bits[5] = {};           // bitmaps of nwords words: 3 rotating frontiers,
-                                        // visited (BFS) / far set (SSSP), scratch flags
+                                        // 2 scratch (symmetry-check flags)
+    uint8_t *d_flags[2] = {};           // frontier byte flags (npairs * 64 bytes)
+    uint8_t *d_far = nullptr;           // SSSP far-set byte flags
     uint32_t nwords = 0;                // 2 * npairs
     uint32_t npairs = 0;                // ceil(V / 64)
     uint32_t *d_queue[2] = {};          // sharded S queues (kShards x kQueueCap)
@@ -149,6 +153,13 @@ struct ptg_graph {
     // Per-launch timing of the main kernel (PTG_FLAG_TIMING).
     std::vector<hipEvent_t> tev;  // pairs: [2i] before, [2i+1] after launch i
     uint32_t tev_used = 0;
+
+    // Degree-ordered relabeling (relabel.cpp): device arrays are in new ids.
+    bool relabeled = false;
+    std::vector<uint32_t> h_perm;       // old id -> new id (host copy for sources)
+    uint32_t *d_perm = nullptr;         // old id -> new id
+    uint32_t *d_inv = nullptr;          // new id -> old id
+    uint32_t *d_val2 = nullptr;         // V: results mapped back to old ids
 
     int symmetric = -1;  // -1 unknown, 0/1 once checked
     uint64_t device_bytes = 0;
@@ -179,6 +190,8 @@ int check_symmetric(ptg_graph *g, int *sym);
 int ensure_in_edges(ptg_graph *g);
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,
                 uint64_t *edges);
+// dst[v] = src[perm[v]] for v < V (new-id results back to old ids).
+int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst);
 
 // Brackets main-kernel launches with HIP events on the graph's stream when
 // PTG_FLAG_TIMING is set; collect() sums the pairs after the stream is idle.
@@ -186,9 +199,20 @@ struct LaunchTimer {
     ptg_graph *g;
     bool on;
     explicit LaunchTimer(ptg_graph *gr);
-    void before();
-    void after();
+    // Event pair for the next timed launch, or false when timing is off.
+    bool next(hipEvent_t *start, hipEvent_t *stop);
     int collect(ptg_stats *st);
+
+    // Launches `kernel`; when timing is on, the dispatch itself records its
+    // start and stop (hipExtLaunchKernelGGL), which adds no stream packets.
+    template <class K, class... A>
+    void launch(K kernel, dim3 grid, dim3 block, A... args) {
+        hipEvent_t a, b;
+        if (next(&a, &b))
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, g->stream, a, b, 0, args...);
+        else
+            hipLaunchKernelGGL(kernel, grid, block, 0, g->stream, args...);
+    }
 };
 
 }  // namespace ptg

@@ -1,0 +1,348 @@
+// Round machinery shared by the BFS, SSSP and label-propagation kernels.
+//
+// Replaces ptGraph's per-iteration pipeline (setLabeling -> thrust::reduce ->
+// thrust::exclusive_scan -> setActiveNodeArray -> setLabelDefault -> relax,
+// ptGraph/main.cu:1094-1106, kernels ptGraph/gpu_kernels.cu:3-172) by two
+// persistent-grid kernels per round: a pre-pass that turns the frontier into
+// work lists, and the round kernel that expands it.
+//
+// A round's input frontier (ring entry r % kRing) comes in one of three forms:
+//   * queue:  small frontiers. The producer appended every vertex once (its
+//             bit in the frontier bitmap dedupes): degree <= sd to a sharded
+//             S queue, larger ones to a sharded chunk queue.
+//   * flags:  large frontiers. The producer only stored a byte per vertex
+//             (plain stores, no atomics: duplicates are harmless); the
+//             pre-pass packs the bytes into the bitmap, counts the frontier
+//             and fills the chunk queue.
+//   * bitmap: the producer wrote whole bitmap words itself (bottom-up BFS,
+//             SSSP far-set splits, seeds) and counted exactly; the pre-pass
+//             fills the chunk queue.
+// The chunk queue holds vertices of degree > sd as ceil(d / kWaveChunk)
+// chunks; wave w expands the contiguous chunk range [C*w/W, C*(w+1)/W), so a
+// hub's list is spread over many waves. Degree <= sd vertices are expanded one
+// per lane from the S queue or straight from bitmap tiles of 64 vertices.
+// Counters are summed per workgroup and added to 1 of 16 shards (a single hot
+// address serialises atomics: measured 11 ns each on MI355X).
+#pragma once
+
+#include "device_util.h"
+
+namespace ptg {
+
+constexpr uint32_t kModeFlags = 2;
+
+struct RoundEnv {
+    RoundCtr *ring;
+    uint32_t r;                   // round index
+    const uint32_t *off, *col;
+    uint32_t *fin, *fout, *fclr;  // frontier bitmaps: read, write, clear
+    uint8_t *flin, *flout;        // frontier byte flags: read (pre-pass), write
+    uint32_t *qin, *qout;         // sharded S queues read / written
+    uint32_t *cqin, *cbin;        // sharded chunk queue read (ids, first chunks)
+    uint32_t *cqout, *cbout;      // sharded chunk queue written
+    uint32_t cq_cap;              // chunk-queue entries per shard
+    uint32_t V, npairs, nwords;
+    uint32_t sd;                  // small degree threshold
+    uint32_t qcap;                // frontier edges up to which the output is queued
+};
+
+// Per-thread tallies, summed per workgroup at the end of a round.
+struct Acc {
+    uint32_t found = 0;
+    uint32_t far = 0;
+    uint64_t edges = 0;
+};
+
+// Output side of a round.
+struct Emit {
+    uint32_t *q;                 // this workgroup's shard of the output S queue
+    uint32_t *qn;                // its count
+    uint32_t *cq, *cb;           // this workgroup's shard of the output chunk queue
+    unsigned long long *cpack;   // its packed count
+    uint32_t *any;               // set to 1 when anything is emitted
+    uint32_t mode;               // kModeQueue, kModeFlags or kModeBitmap
+    uint32_t sd;
+};
+
+__device__ __forceinline__ uint32_t wave_chunks(uint32_t d) { return (d + kWaveChunk - 1) / kWaveChunk; }
+
+__device__ __forceinline__ Emit make_emit(const RoundEnv &e, RoundCtr *out, uint32_t mode) {
+    const uint32_t shard = blockIdx.x % kShards;
+    return Emit{e.qout + (uint64_t)shard * kQueueCap, &out->qn[shard],
+                e.cqout + (uint64_t)shard * e.cq_cap, e.cbout + (uint64_t)shard * e.cq_cap,
+                &out->cpack[shard], &out->any, mode, e.sd};
+}
+
+// Marks the output non-empty (one plain store per wave that has pred).
+__device__ __forceinline__ void mark_any(const Emit &em, bool pred) {
+    const uint64_t m = ballot(pred);
+    if (m && lane_id() == lowest_lane(m)) *em.any = 1u;
+}
+
+// Appends chunk-queue entries (d > sd) with one packed atomic per wave.
+__device__ __forceinline__ void append_chunks(uint32_t *cq, uint32_t *cb, unsigned long long *cpack, bool pc,
+                                              uint32_t w, uint32_t d) {
+    const uint64_t mc = ballot(pc);
+    if (!mc) return;
+    const uint32_t nch = pc ? wave_chunks(d) : 0u;
+    const uint32_t incl = wave_incl_scan_u32(nch);
+    const uint32_t total = read_lane(incl, 63);
+    const uint32_t leader = lowest_lane(mc);
+    unsigned long long old = 0;
+    if (lane_id() == leader)
+        old = atomicAdd(cpack, ((unsigned long long)__popcll(mc) << kLCountShift) | (unsigned long long)total);
+    old = read_lane64(old, leader);
+    if (pc) {
+        const uint32_t slot = (uint32_t)(old >> kLCountShift) + rank_below(mc);
+        cq[slot] = w;
+        cb[slot] = (uint32_t)(old & kLChunkMask) + (incl - nch);
+    }
+}
+
+// Records vertex w (out-degree d), newly in the next frontier, for every lane
+// with pred, in queue or bitmap mode (the caller set its bitmap bit). Counts
+// it, and in queue mode appends it to the S or chunk queue. Wave-uniform.
+__device__ __forceinline__ void emit(const Emit &em, bool pred, uint32_t w, uint32_t d, Acc &acc) {
+    if (pred) {
+        acc.found++;
+        acc.edges += d;
+    }
+    mark_any(em, pred);
+    if (em.mode != kModeQueue) return;
+    wave_append(em.q, em.qn, pred && d > 0 && d <= em.sd, w);
+    append_chunks(em.cq, em.cb, em.cpack, pred && d > em.sd, w, d);
+}
+
+struct InSummary {
+    uint32_t found;
+    uint32_t far;
+    uint64_t edges;
+};
+
+__device__ __forceinline__ InSummary summarize(const RoundCtr *in) {
+    InSummary s{0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < kShards; i++) {
+        s.found += in->found_sh[i];
+        s.far += in->far_sh[i];
+        s.edges += in->edges_sh[i];
+    }
+    return s;
+}
+
+// Zeroes the ring entry two rounds ahead (done by every round, active or
+// not, so rounds launched past the end of a search always find zeros).
+__device__ __forceinline__ void clear_ring_ahead(const RoundEnv &e) {
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(RoundCtr) / 4)
+        reinterpret_cast<uint32_t *>(e.ring + (e.r + 2) % kRing)[threadIdx.x] = 0u;
+}
+
+// Zeroes the frontier bitmap the round after next will write.
+__device__ __forceinline__ void clear_bitmap_ahead(const RoundEnv &e) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < e.nwords; i += stride) e.fclr[i] = 0u;
+}
+
+// Adds the workgroup's tallies to shard blockIdx % kShards of `out`.
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void add_tallies(RoundCtr *out, const Acc &acc) {
+    __shared__ uint32_t s_found[kWavesPerBlock], s_far[kWavesPerBlock];
+    __shared__ unsigned long long s_edges[kWavesPerBlock];
+    const uint32_t f = (uint32_t)wave_sum_u64(acc.found);
+    const uint32_t fa = (uint32_t)wave_sum_u64(acc.far);
+    const uint64_t ed = wave_sum_u64(acc.edges);
+    const uint32_t wib = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_found[wib] = f;
+        s_far[wib] = fa;
+        s_edges[wib] = ed;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tf = 0, tfa = 0;
+        unsigned long long te = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) {
+            tf += s_found[w];
+            tfa += s_far[w];
+            te += s_edges[w];
+        }
+        const uint32_t shard = blockIdx.x % kShards;
+        if (tf) atomicAdd(&out->found_sh[shard], tf);
+        if (tfa) atomicAdd(&out->far_sh[shard], tfa);
+        if (te) atomicAdd(&out->edges_sh[shard], te);
+    }
+}
+
+// Lane-per-vertex expansion of the tile's vertices of degree 1..sd (`mem`
+// says whether the lane holds a frontier vertex). Wave-uniform call.
+template <class Op>
+__device__ __forceinline__ void process_tile(const RoundEnv &e, bool mem, uint32_t v, Op &op, const Emit &em,
+                                             Acc &acc) {
+    uint32_t b = 0, d = 0;
+    if (mem) {
+        b = e.off[v];
+        d = e.off[v + 1] - b;
+    }
+    const bool s = mem && d > 0 && d <= e.sd;
+    const uint32_t ds = s ? d : 0u;
+    const uint32_t sv = s ? op.vertex(v) : 0u;
+    const uint32_t md = wave_max_u32(ds);
+    for (uint32_t k = 0; k < md; k++) op.edge(k < ds, sv, (uint64_t)b + k, em, acc);
+}
+
+// Walks the chunk queue: wave gw takes the contiguous global chunk range
+// [C*gw/nw, C*(gw+1)/nw), finds its first entry by binary search and then
+// steps through chunks (kWaveChunk edges, 64 lanes per step).
+template <class Op>
+__device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr *in, Op &op, const Emit &em,
+                                              Acc &acc, uint32_t gw, uint32_t nw) {
+    const uint32_t lane = lane_id();
+    uint32_t cpre[kShards + 1];
+    cpre[0] = 0;
+#pragma unroll
+    for (int s = 0; s < kShards; s++) cpre[s + 1] = cpre[s] + (uint32_t)(in->cpack[s] & kLChunkMask);
+    const uint32_t C = cpre[kShards];
+    const uint32_t c0 = (uint32_t)((uint64_t)C * gw / nw), c1 = (uint32_t)((uint64_t)C * (gw + 1) / nw);
+    if (c0 >= c1) return;
+    int s = 0;
+    while (s < kShards - 1 && c0 >= cpre[s + 1]) s++;
+    const uint32_t *ids = e.cqin + (uint64_t)s * e.cq_cap;
+    const uint32_t *bases = e.cbin + (uint64_t)s * e.cq_cap;
+    uint32_t n = (uint32_t)(in->cpack[s] >> kLCountShift);
+    const uint32_t local = c0 - cpre[s];
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bases[mid] <= local) lo = mid; else hi = mid;
+    }
+    uint32_t idx = lo;
+    uint32_t v = ids[idx];
+    uint32_t vb = e.off[v], vd = e.off[v + 1] - vb;
+    uint32_t k = local - bases[idx];
+    uint32_t sv = op.vertex(v);
+    for (uint32_t c = c0; c < c1; c++) {
+        while (k >= wave_chunks(vd)) {  // next entry, possibly in the next shard
+            k = 0;
+            if (++idx >= n) {
+                do {
+                    s++;
+                } while (s < kShards - 1 && cpre[s + 1] == cpre[s]);
+                ids = e.cqin + (uint64_t)s * e.cq_cap;
+                n = (uint32_t)(in->cpack[s] >> kLCountShift);
+                idx = 0;
+            }
+            v = ids[idx];
+            vb = e.off[v];
+            vd = e.off[v + 1] - vb;
+            sv = op.vertex(v);
+        }
+        const uint32_t b = k * kWaveChunk;
+        const uint32_t end = min(vd, b + kWaveChunk);
+        for (uint32_t t = b; t < end; t += 64) op.edge(t + lane < end, sv, (uint64_t)vb + t + lane, em, acc);
+        k++;
+    }
+}
+
+// Expands the frontier of round e.r: the chunk queue, then the degree <= sd
+// vertices from the S queue or the bitmap.
+template <class Op>
+__device__ __forceinline__ void expand_frontier(const RoundEnv &e, const RoundCtr *in, Op &op, const Emit &em,
+                                                Acc &acc) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    expand_chunks(e, in, op, em, acc, gw, nw);
+    if (in->mode == kModeQueue) {
+        uint32_t pre[kShards + 1];
+        pre[0] = 0;
+#pragma unroll
+        for (int s = 0; s < kShards; s++) pre[s + 1] = pre[s] + in->qn[s];
+        const uint32_t total = pre[kShards];
+        for (uint32_t t = gw * 64; t < total; t += nw * 64) {
+            const uint32_t i = t + lane;
+            const bool valid = i < total;
+            uint32_t v = 0;
+            if (valid) {
+                int s = 0;
+                while (s < kShards - 1 && i >= pre[s + 1]) s++;
+                v = e.qin[(uint64_t)s * kQueueCap + (i - pre[s])];
+            }
+            process_tile(e, valid, v, op, em, acc);
+        }
+    } else {
+        for (uint32_t p = gw; p < e.npairs; p += nw) {
+            const uint32_t bits = e.fin[2 * p + (lane >> 5)];
+            const bool mem = (bits >> (lane & 31)) & 1u;
+            if (ballot(mem) == 0) continue;
+            process_tile(e, mem, p * 64 + lane, op, em, acc);
+        }
+    }
+}
+
+// ---- host side ---------------------------------------------------------------
+
+RoundEnv make_env(ptg_graph *g, uint32_t r);
+// Clears the frontier bitmaps, byte flags and counter ring before a search.
+int reset_rounds(ptg_graph *g);
+// Seeds round 0 with one source of value src_value (queue mode).
+int launch_seed(ptg_graph *g, uint32_t src, uint32_t src_value, uint32_t *val, uint32_t thr);
+// Pre-pass of round r (a no-op for queue-mode frontiers).
+int launch_prepare(ptg_graph *g, const RoundEnv &e);
+int read_ring(ptg_graph *g, uint32_t from, uint32_t count);
+
+struct HostSummary {
+    uint32_t found, far;
+    uint64_t edges;
+};
+
+inline HostSummary host_summary(const RoundCtr &c) {
+    HostSummary s{0, 0, 0};
+    for (int i = 0; i < kShards; i++) {
+        s.found += c.found_sh[i];
+        s.far += c.far_sh[i];
+        s.edges += c.edges_sh[i];
+    }
+    return s;
+}
+
+// Launches rounds (pre-pass + round kernel, via launch(r)) in growing
+// batches until a ring entry is idle. After a batch that ended with round r-1,
+// entries 0..r-1 are complete and entry r holds only its header and `any`
+// flag, so idleness is judged on every entry but counts on complete ones.
+// Returns the number of rounds that did work and the sums of their counts.
+template <class Launch, class Idle>
+int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t *rounds_out, uint64_t *found_sum,
+                 uint64_t *edges_sum) {
+    uint32_t r = 0, batch = 2, rounds = 0;
+    uint64_t fs = 0, es = 0;
+    uint32_t checked = 0;  // ring entries [0, checked) already summed
+    for (;;) {
+        for (uint32_t k = 0; k < batch; k++, r++) {
+            int rc = launch(r);
+            if (rc) return rc;
+        }
+        int rc = read_ring(g, checked, r + 1 - checked);
+        if (rc) return rc;
+        bool done = false;
+        for (; checked <= r; checked++) {
+            const RoundCtr &c = g->h_ring[checked % kRing];
+            if (idle(c)) {
+                done = true;
+                break;
+            }
+            if (checked == r) break;  // not prepared yet: summed after the next batch
+            const HostSummary s = host_summary(c);
+            fs += s.found;
+            es += s.edges;
+            rounds++;
+        }
+        if (done) break;
+        if (batch < 16) batch *= 2;
+    }
+    *rounds_out = rounds;
+    if (found_sum) *found_sum = fs;
+    if (edges_sum) *edges_sum = es;
+    return PTG_OK;
+}
+
+}  // namespace ptg

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ptg_internal.h"
+#include "relabel.h"
 
 namespace ptg {
 
@@ -124,30 +125,27 @@ LaunchTimer::LaunchTimer(ptg_graph *gr) : g(gr), on((gr->cfg.flags & PTG_FLAG_TI
     g->tev_used = 0;
 }
 
-void LaunchTimer::before() {
-    if (!on) return;
+bool LaunchTimer::next(hipEvent_t *start, hipEvent_t *stop) {
+    if (!on) return false;
     const size_t need = 2ull * (g->tev_used + 1);
     if (g->tev.size() < need) {
         if (g->tev.size() >= 2ull * 65536) {  // cap: stop timing very long queries
             on = false;
-            return;
+            return false;
         }
         while (g->tev.size() < need) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) {
                 on = false;
-                return;
+                return false;
             }
             g->tev.push_back(e);
         }
     }
-    (void)hipEventRecord(g->tev[2 * g->tev_used], g->stream);
-}
-
-void LaunchTimer::after() {
-    if (!on) return;
-    (void)hipEventRecord(g->tev[2 * g->tev_used + 1], g->stream);
+    *start = g->tev[2 * g->tev_used];
+    *stop = g->tev[2 * g->tev_used + 1];
     g->tev_used++;
+    return true;
 }
 
 int LaunchTimer::collect(ptg_stats *st) {
@@ -176,7 +174,12 @@ static void destroy(ptg_graph *g) {
         free_static_bins(g, &g->in.bins);
     }
     gfree(g, g->d_val);
+    gfree(g, g->d_val2);
+    gfree(g, g->d_perm);
+    gfree(g, g->d_inv);
     for (auto *b : g->d_bits) gfree(g, b);
+    for (auto *f : g->d_flags) gfree(g, f);
+    gfree(g, g->d_far);
     gfree(g, g->d_qmem);
     gfree(g, g->d_ring);
     gfree(g, g->d_acc);
@@ -245,19 +248,31 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
         TRY(fail(PTG_ERR_HIP, "hipEventCreate"));
 
-    // CSR: V+1 offsets built from the reference's V-entry layout.
-    std::vector<uint32_t> h_off((size_t)V + 1);
-    memcpy(h_off.data(), ptr, sizeof(uint32_t) * V);
-    h_off[V] = (uint32_t)E;
+    // CSR in HBM: V+1 offsets, relabeled by degree unless asked not to.
+    std::vector<uint32_t> h_off;
+    Relabeled rl;
+    const bool relabel = !(c.flags & PTG_FLAG_NO_RELABEL);
+    const uint32_t *up_col = cols, *up_w = weights;
+    if (relabel) {
+        relabel_by_degree(V, E, ptr, cols, weights, wedges, &rl);
+        h_off.swap(rl.off);
+        up_col = rl.col.data();
+        up_w = g->weighted ? rl.w.data() : nullptr;
+        g->relabeled = true;
+    } else {
+        h_off.resize((size_t)V + 1);
+        memcpy(h_off.data(), ptr, sizeof(uint32_t) * V);
+        h_off[V] = (uint32_t)E;
+    }
     TRY(galloc(g, (void **)&g->out.off, sizeof(uint32_t) * ((uint64_t)V + 1)));
     TRY(galloc(g, (void **)&g->out.col, sizeof(uint32_t) * E));
     if (g->weighted) TRY(galloc(g, (void **)&g->d_w, sizeof(uint32_t) * E));
     if (hipMemcpy(g->out.off, h_off.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice) != hipSuccess)
         TRY(fail(PTG_ERR_HIP, "hipMemcpy(offsets)"));
-    if (cols) {
-        if (E && hipMemcpy(g->out.col, cols, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
+    if (up_col) {
+        if (E && hipMemcpy(g->out.col, up_col, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
             TRY(fail(PTG_ERR_HIP, "hipMemcpy(cols)"));
-        if (weights && E && hipMemcpy(g->d_w, weights, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
+        if (up_w && E && hipMemcpy(g->d_w, up_w, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
             TRY(fail(PTG_ERR_HIP, "hipMemcpy(weights)"));
     } else if (E) {
         // AoS EdgeWithWeight -> SoA columns + weights, in host blocks.
@@ -274,6 +289,15 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
                 TRY(fail(PTG_ERR_HIP, "hipMemcpy(edges)"));
         }
     }
+    if (relabel) {
+        TRY(galloc(g, (void **)&g->d_perm, sizeof(uint32_t) * V));
+        TRY(galloc(g, (void **)&g->d_inv, sizeof(uint32_t) * V));
+        TRY(galloc(g, (void **)&g->d_val2, sizeof(uint32_t) * V));
+        if (hipMemcpy(g->d_perm, rl.perm.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(g->d_inv, rl.inv.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice) != hipSuccess)
+            TRY(fail(PTG_ERR_HIP, "hipMemcpy(permutation)"));
+        g->h_perm.swap(rl.perm);
+    }
     TRY(build_static_bins(g, h_off.data(), V, &g->out.bins));
 
     // Query state.
@@ -281,6 +305,8 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     g->nwords = 2 * g->npairs;
     TRY(galloc(g, (void **)&g->d_val, sizeof(uint32_t) * V));
     for (auto &b : g->d_bits) TRY(galloc(g, (void **)&b, sizeof(uint32_t) * g->nwords));
+    for (auto &f : g->d_flags) TRY(galloc(g, (void **)&f, 64ull * g->npairs));
+    TRY(galloc(g, (void **)&g->d_far, 64ull * g->npairs));
     // per buffer: sharded S queue and sharded chunk queue (ids + chunk bases)
     g->cq_cap = chunk_queue_cap(V, g->grid);
     const uint64_t qwords = (uint64_t)kShards * kQueueCap;
@@ -297,6 +323,10 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     TRY(galloc(g, (void **)&g->d_acc, sizeof(unsigned long long) * 4));
     if (hipHostMalloc((void **)&g->h_ring, sizeof(RoundCtr) * kRing, hipHostMallocDefault) != hipSuccess)
         TRY(fail(PTG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters)"));
+    {
+        int sym = 0;
+        TRY(check_symmetric(g, &sym));
+    }
 #undef TRY
     *out = g;
     return PTG_OK;
@@ -306,18 +336,35 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
 
 using namespace ptg;
 
-// Runs `fn` into a device buffer: the caller's when it is device memory,
-// otherwise the graph's scratch followed by a copy back to the host.
+// Runs `fn` (which writes a V-entry result in device ids into the buffer it
+// is given) and delivers the result in the caller's ids to `user`, which may
+// be device or host memory.
 template <class T, class F>
-static int with_output(ptg_graph *g, T *user, uint64_t n, T *scratch, F fn) {
+static int with_output(ptg_graph *g, T *user, F fn) {
     if (!user) return fail(PTG_ERR_INVALID_ARG, "null output pointer");
-    if (is_device_pointer(user)) return fn(user);
-    int rc = fn(scratch);
-    if (rc) return rc;
-    PTG_HIP(hipMemcpyAsync(user, scratch, n * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+    const bool dev = is_device_pointer(user);
+    uint32_t *target = reinterpret_cast<uint32_t *>(user);
+    int rc;
+    if (!g->relabeled) {
+        if (dev) return fn(reinterpret_cast<T *>(target));
+        if ((rc = fn(reinterpret_cast<T *>(g->d_val)))) return rc;
+        target = g->d_val;
+    } else {
+        if ((rc = fn(reinterpret_cast<T *>(g->d_val)))) return rc;
+        uint32_t *dst = dev ? target : g->d_val2;
+        if ((rc = unpermute(g, g->d_val, dst))) return rc;
+        if (dev) {
+            PTG_HIP(hipStreamSynchronize(g->stream));
+            return PTG_OK;
+        }
+        target = g->d_val2;
+    }
+    PTG_HIP(hipMemcpyAsync(user, target, (uint64_t)g->V * sizeof(T), hipMemcpyDeviceToHost, g->stream));
     PTG_HIP(hipStreamSynchronize(g->stream));
     return PTG_OK;
 }
+
+static uint32_t map_source(const ptg_graph *g, uint32_t src) { return g->relabeled ? g->h_perm[src] : src; }
 
 extern "C" {
 
@@ -435,7 +482,8 @@ uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const pt
     auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
     uint64_t b = al(4 * v1) + al(4 * E) + (weighted ? al(4 * E) : 0);  // CSR
     b += al(4 * (uint64_t)V + 4 * (2 * lcap) + 64);                    // static bins
-    b += al(4 * (uint64_t)V) + 5 * al(4 * nwords);                     // values + bitmaps
+    b += al(4 * (uint64_t)V) + 5 * al(4 * nwords) + 3 * al(32 * nwords); // values, bitmaps, byte flags
+    if (!(c.flags & PTG_FLAG_NO_RELABEL)) b += 3 * al(4 * (uint64_t)V); // permutation, inverse, results
     b += al(4 * 2 * ((uint64_t)kShards * kQueueCap + 2ull * kShards * chunk_queue_cap(V, grid)));  // queues
     b += al(sizeof(RoundCtr) * kRing) + al(32);                        // counter ring, accumulators
     return b;
@@ -453,27 +501,28 @@ int ptg_graph_is_symmetric(ptg_graph *g) {
 int ptg_bfs(ptg_graph *g, uint32_t src, uint32_t *levels_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "bfs: source >= V");
-    return with_output(g, levels_out, g->V, g->d_val, [&](uint32_t *o) { return run_bfs(g, src, o, st); });
+    const uint32_t s = map_source(g, src);
+    return with_output(g, levels_out, [&](uint32_t *o) { return run_bfs(g, s, o, st); });
 }
 
 int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "sssp: source >= V");
     if (!g->weighted) return fail(PTG_ERR_UNSUPPORTED, "sssp: graph has no weights");
-    return with_output(g, dist_out, g->V, g->d_val, [&](uint32_t *o) { return run_sssp(g, src, o, st); });
+    const uint32_t s = map_source(g, src);
+    return with_output(g, dist_out, [&](uint32_t *o) { return run_sssp(g, s, o, st); });
 }
 
 int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
-    return with_output(g, labels_out, g->V, g->d_val, [&](uint32_t *o) { return run_cc(g, o, st); });
+    return with_output(g, labels_out, [&](uint32_t *o) { return run_cc(g, o, st); });
 }
 
 int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (iterations < 0 || !(damping >= 0.0f && damping <= 1.0f))
         return fail(PTG_ERR_INVALID_ARG, "pagerank: need iterations >= 0 and 0 <= damping <= 1");
-    return with_output(g, ranks_out, g->V, (float *)g->d_val,
-                       [&](float *o) { return run_pagerank(g, damping, iterations, o, st); });
+    return with_output(g, ranks_out, [&](float *o) { return run_pagerank(g, damping, iterations, o, st); });
 }
 
 }  // extern "C"

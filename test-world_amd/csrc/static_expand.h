@@ -1,0 +1,56 @@
+// Expansion over a fixed vertex set binned by degree (static bins built when
+// the graph is created): CC hooking and the symmetry checks use it.
+#pragma once
+
+#include "device_util.h"
+
+namespace ptg {
+
+// Walks the L (hub chunks of kChunkEdges, one workgroup each), M (one wave
+// per vertex) and S (one lane per vertex) bins and calls op.edge() from
+// wave-uniform control flow.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_expand(Frontier in, const uint32_t *__restrict__ off, Op op) {
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    const unsigned long long lp = in.ctr->l_pack;
+    const uint32_t nl = (uint32_t)(lp >> kLCountShift);
+    const uint32_t nch = (uint32_t)(lp & kLChunkMask);
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        uint32_t lo = 0, hi = nl;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (in.lb[mid] <= c) lo = mid; else hi = mid;
+        }
+        const uint32_t v = in.ql[lo];
+        const uint64_t b = (uint64_t)off[v] + (uint64_t)(c - in.lb[lo]) * kChunkEdges;
+        const uint64_t e = min((uint64_t)off[v + 1], b + kChunkEdges);
+        const uint32_t sv = op.vertex(v);
+        for (uint64_t base = b + wib * 64; base < e; base += kBlock) op.edge(base + lane < e, sv, base + lane);
+    }
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const uint32_t nm = in.ctr->n_m;
+    for (uint32_t i = gw; i < nm; i += nw) {
+        const uint32_t v = in.qm[i];
+        const uint64_t b = off[v], e = off[v + 1];
+        const uint32_t sv = op.vertex(v);
+        for (uint64_t base = b; base < e; base += 64) op.edge(base + lane < e, sv, base + lane);
+    }
+    const uint32_t ns = in.ctr->n_s;
+    for (uint64_t base = (uint64_t)gw * 64; base < ns; base += (uint64_t)nw * 64) {
+        const uint64_t i = base + lane;
+        const bool valid = i < ns;
+        uint32_t b = 0, d = 0, sv = 0;
+        if (valid) {
+            const uint32_t v = in.qs[i];
+            b = off[v];
+            d = off[v + 1] - b;
+            sv = op.vertex(v);
+        }
+        const uint32_t md = wave_max_u32(d);
+        for (uint32_t j = 0; j < md; j++) op.edge(j < d, sv, (uint64_t)b + j);
+    }
+}
+
+}  // namespace ptg

@@ -28,6 +28,7 @@ ERR_IO = -9
 
 FLAG_NO_VALIDATE = 0x1
 FLAG_TIMING = 0x2
+FLAG_NO_RELABEL = 0x4
 
 BFS_UNREACHED = 0xFFFFFFFF  # ptGraph/main.cu:1068
 SSSP_SENTINEL_SHARE = 0xFFFFFFFE  # ptGraph/main.cu:584

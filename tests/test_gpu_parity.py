@@ -26,6 +26,8 @@ CONFIGS = {
     "top_down": {"td_to_bu": 1e-9},
     "one_block": {"grid_blocks": 1},
     "odd_grid": {"grid_blocks": 7, "small_degree": 3, "large_degree": 64},
+    "no_relabel": {"flags": pg.FLAG_NO_RELABEL},
+    "no_relabel_bu": {"flags": pg.FLAG_NO_RELABEL, "td_to_bu": 1e9, "bu_to_td": 1e9, "small_degree": 2},
 }
 
 
@@ -183,12 +185,13 @@ def random_directed(rng, V, E, wmax):
     return ptr, dst.astype(np.uint32), w
 
 
+@pytest.mark.parametrize("flags", [0, pg.FLAG_NO_RELABEL], ids=["relabel", "no_relabel"])
 @pytest.mark.parametrize("seed", [1, 2])
-def test_directed_graph(seed):
+def test_directed_graph(seed, flags):
     rng = np.random.default_rng(seed)
     ptr, col, w = random_directed(rng, 3000, 9000, 9)
-    say("case directed seed %d" % seed)
-    with pg.Graph(ptr, col, w) as g:
+    say("case directed seed %d flags %d" % (seed, flags))
+    with pg.Graph(ptr, col, w, config=pg.default_config(flags=flags)) as g:
         assert not g.is_symmetric()
         for s in (0, 17, 2999):
             lv, _ = g.bfs(s)

@@ -94,12 +94,13 @@ __global__ __launch_bounds__(kBlock) void k_init_keys(uint32_t V, const uint32_t
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) val[v] = key ? key[v] : v;
 }
 
-// Number of v whose label is its own key, one per component (added to 1 of
-// 4 accumulators).
+// Number of v whose label is its own key, one per component (summed per
+// workgroup, then added to 1 of 16 shards).
 __global__ __launch_bounds__(kBlock) void k_count_roots(uint32_t V, const uint32_t *__restrict__ label,
                                                         const uint32_t *__restrict__ key, unsigned long long *out) {
-    const uint32_t lane = lane_id();
-    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    __shared__ unsigned long long s_c[kWavesPerBlock];
+    const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     uint64_t cnt = 0;
     for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64) {
@@ -107,19 +108,27 @@ __global__ __launch_bounds__(kBlock) void k_count_roots(uint32_t V, const uint32
         cnt += (v < V && label[v] == (key ? key[v] : (uint32_t)v));
     }
     cnt = wave_sum_u64(cnt);
-    if (lane == 0 && cnt) atomicAdd(&out[blockIdx.x % 4], (unsigned long long)cnt);
+    if (lane == 0) s_c[wib] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) c += s_c[w];
+        if (c) atomicAdd(&out[blockIdx.x % kShards], c);
+    }
 }
 
 static int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots) {
     unsigned long long *acc = g->d_acc;
-    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 8, 0u);
+    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 2 * kShards, 0u);
     if (rc) return rc;
     k_count_roots<<<g->grid, kBlock, 0, g->stream>>>(g->V, label, g->d_inv, acc);
     PTG_HIP(hipGetLastError());
-    unsigned long long h[4] = {0, 0, 0, 0};
+    unsigned long long h[kShards];
     PTG_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, g->stream));
     PTG_HIP(hipStreamSynchronize(g->stream));
-    *roots = (uint32_t)(h[0] + h[1] + h[2] + h[3]);
+    uint64_t c = 0;
+    for (int s = 0; s < kShards; s++) c += h[s];
+    *roots = (uint32_t)c;
     return PTG_OK;
 }
 

@@ -99,11 +99,13 @@ int check_symmetric(ptg_graph *g, int *sym) {
 // Reached count and m_r = sum of out-degrees of reached vertices.
 // ---------------------------------------------------------------------------
 
+// Sums per workgroup, then adds to 1 of 16 shards of (count, edges) pairs.
 __global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32_t *__restrict__ val,
                                                         uint32_t unreached, const uint32_t *__restrict__ off,
                                                         unsigned long long *out) {
-    const uint32_t lane = lane_id();
-    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    __shared__ unsigned long long s_c[kWavesPerBlock], s_e[kWavesPerBlock];
+    const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     uint64_t cnt = 0, edges = 0;
     for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64) {
@@ -115,25 +117,42 @@ __global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32
     }
     cnt = wave_sum_u64(cnt);
     edges = wave_sum_u64(edges);
-    if (lane == 0 && cnt) {
-        const uint32_t s = 2 * (blockIdx.x % 2);
-        atomicAdd(&out[s], (unsigned long long)cnt);
-        atomicAdd(&out[s + 1], (unsigned long long)edges);
+    if (lane == 0) {
+        s_c[wib] = cnt;
+        s_e[wib] = edges;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0, e = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) {
+            c += s_c[w];
+            e += s_e[w];
+        }
+        const uint32_t s = 2 * (blockIdx.x % kShards);
+        if (c) {
+            atomicAdd(&out[s], c);
+            atomicAdd(&out[s + 1], e);
+        }
     }
 }
 
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached, uint64_t *edges) {
     if (!reached && !edges) return PTG_OK;
     unsigned long long *acc = g->d_acc;
-    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 8, 0u);
+    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 4 * kShards, 0u);
     if (rc) return rc;
     k_reach_stats<<<g->grid, kBlock, 0, g->stream>>>(g->V, val, unreached, g->out.off, acc);
     PTG_HIP(hipGetLastError());
-    unsigned long long h[4];
+    unsigned long long h[2 * kShards];
     PTG_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, g->stream));
     PTG_HIP(hipStreamSynchronize(g->stream));
-    if (reached) *reached = (uint32_t)(h[0] + h[2]);
-    if (edges) *edges = h[1] + h[3];
+    uint64_t c = 0, e = 0;
+    for (int s = 0; s < kShards; s++) {
+        c += h[2 * s];
+        e += h[2 * s + 1];
+    }
+    if (reached) *reached = (uint32_t)c;
+    if (edges) *edges = e;
     return PTG_OK;
 }
 

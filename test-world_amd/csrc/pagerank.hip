@@ -7,9 +7,10 @@
 //   r'(v) = (1-d)/V + d * ( sum_{u->v} r(u)/outdeg(u) + D/V ),
 //   D     = sum of r(u) over vertices with outdeg(u) = 0.
 // Per iteration, a gather kernel over static in-degree bins (hub lists split
-// into chunks) and a one-workgroup kernel that combines the hub chunks and
-// the dangling mass; the kernel boundary orders them, so no inter-workgroup
-// fences or counters are needed. Every sum is a (hi, lo) pair updated with
+// into chunks) and a combine kernel that finishes the hubs; the next gather
+// sums the dangling mass from both kernels' per-workgroup partials. Kernel
+// boundaries order all of it, so no inter-workgroup fences or counters are
+// needed. Every sum is a (hi, lo) pair updated with
 // Knuth's TwoSum, and every reduction runs in a fixed order, so results are
 // bit-identical run to run.
 #include <string.h>
@@ -70,11 +71,13 @@ struct PrArgs {
     const float *cin;               // r(u)/outdeg(u) of the previous iteration
     float *cout;                    // same for this iteration
     float *rank_out;                // ranks (last iteration only) or null
-    const float *dang_in;           // (hi, lo) of D for the previous iteration
-    float *dang_out;
-    float *part;                    // per-workgroup dangling partials (hi, lo)
+    const float *d0;                // (hi, lo) of the dangling mass D for iteration 0, or null
+    const float *part_in;           // previous iteration's dangling partials: gather workgroups
+    const float *cpart_in;          //   and combine workgroups, (hi, lo) each
+    float *part_out, *cpart_out;    // this iteration's partials
+    float *base_out;                // this iteration's base value, for the combine kernel
     float *lpart;                   // hub chunk partials (hi, lo)
-    uint32_t gather_blocks;         // workgroups of the gather kernel
+    uint32_t gather_blocks, combine_blocks;
     float damping, inv_v;
 };
 
@@ -93,18 +96,33 @@ struct PrVertex {
     }
 };
 
-__device__ __forceinline__ float pr_base(const PrArgs &a) {
-    const float dang = a.dang_in[0] + a.dang_in[1];
+// base = (1-d)/V + d * D/V, with D summed from the previous iteration's
+// partials in the same fixed order by every workgroup.
+__device__ __forceinline__ float pr_base(const PrArgs &a, CF *lds) {
+    float dang;
+    if (a.d0) {
+        dang = a.d0[0] + a.d0[1];
+    } else {
+        CF t{0.0f, 0.0f};
+        for (uint32_t j = threadIdx.x; j < a.gather_blocks; j += kBlock)
+            t = cf_merge(t, CF{a.part_in[2 * j], a.part_in[2 * j + 1]});
+        for (uint32_t j = threadIdx.x; j < a.combine_blocks; j += kBlock)
+            t = cf_merge(t, CF{a.cpart_in[2 * j], a.cpart_in[2 * j + 1]});
+        t = block_cf_sum(t, lds);
+        dang = t.hi + t.lo;
+    }
     return (1.0f - a.damping) * a.inv_v + a.damping * (dang * a.inv_v);
 }
 
 // Gathers every vertex's in-neighbour contributions. Hub lists are cut into
 // chunks whose partial sums go to lpart; k_pr_combine finishes those
-// vertices. Each workgroup leaves its share of the dangling mass in part.
+// vertices. Each workgroup leaves its share of the dangling mass in part_out.
 __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
     __shared__ CF lds[kWavesPerBlock];
     CF my_dang{0.0f, 0.0f};
-    const PrVertex pv{a, pr_base(a), my_dang};
+    const float base = pr_base(a, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.base_out[0] = base;
+    const PrVertex pv{a, base, my_dang};
     const uint32_t lane = lane_id();
 
     // Hubs: one workgroup per chunk of kChunkEdges in-edges.
@@ -157,33 +175,36 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
 
     const CF bd = block_cf_sum(my_dang, lds);
     if (threadIdx.x == 0) {
-        a.part[2 * blockIdx.x] = bd.hi;
-        a.part[2 * blockIdx.x + 1] = bd.lo;
+        a.part_out[2 * blockIdx.x] = bd.hi;
+        a.part_out[2 * blockIdx.x + 1] = bd.lo;
     }
 }
 
-// One workgroup: finishes the hub vertices from their chunk partials (in chunk
-// order) and sums the dangling mass of all gather workgroups (in index order).
+// Finishes the hub vertices: one wave per hub sums its chunk partials (lane
+// stride, then a fixed butterfly). Each workgroup leaves the dangling mass of
+// its hubs in cpart_out.
 __global__ __launch_bounds__(kBlock) void k_pr_combine(PrArgs a) {
     __shared__ CF lds[kWavesPerBlock];
     CF my_dang{0.0f, 0.0f};
-    const PrVertex pv{a, pr_base(a), my_dang};
-    const unsigned long long lp = a.bins.ctr->l_pack;
-    const uint32_t nl = (uint32_t)(lp >> kLCountShift);
-    for (uint32_t l = threadIdx.x; l < nl; l += kBlock) {
+    const PrVertex pv{a, a.base_out[0], my_dang};
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const uint32_t nl = (uint32_t)(a.bins.ctr->l_pack >> kLCountShift);
+    for (uint32_t l = gw; l < nl; l += nw) {
         const uint32_t v = a.bins.ql[l];
         const uint32_t first = a.bins.lb[l];
         const uint32_t n = bin_chunks(a.ioff[v + 1] - a.ioff[v]);
         CF t{0.0f, 0.0f};
-        for (uint32_t j = 0; j < n; j++) t = cf_merge(t, CF{a.lpart[2ull * (first + j)], a.lpart[2ull * (first + j) + 1]});
-        pv.finalize(v, t.hi + t.lo);
+        for (uint32_t j = lane; j < n; j += 64)
+            t = cf_merge(t, CF{a.lpart[2ull * (first + j)], a.lpart[2ull * (first + j) + 1]});
+        t = wave_cf_sum(t);
+        if (lane == 0) pv.finalize(v, t.hi + t.lo);
     }
-    for (uint32_t j = threadIdx.x; j < a.gather_blocks; j += kBlock)
-        my_dang = cf_merge(my_dang, CF{a.part[2 * j], a.part[2 * j + 1]});
-    const CF t = block_cf_sum(my_dang, lds);
+    const CF bd = block_cf_sum(my_dang, lds);
     if (threadIdx.x == 0) {
-        a.dang_out[0] = t.hi;
-        a.dang_out[1] = t.lo;
+        a.cpart_out[2 * blockIdx.x] = bd.hi;
+        a.cpart_out[2 * blockIdx.x + 1] = bd.lo;
     }
 }
 
@@ -238,14 +259,22 @@ int ensure_in_edges(ptg_graph *g) {
     return PTG_OK;
 }
 
+constexpr uint32_t kMaxCombineBlocks = 256;
+
+static uint32_t combine_blocks(const ptg_graph *g) {
+    const uint64_t nl = g->in.bins.h_ctr.l_pack >> kLCountShift;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kMaxCombineBlocks, (nl + kWavesPerBlock - 1) / kWavesPerBlock));
+}
+
 static int ensure_pr_state(ptg_graph *g) {
     if (g->d_contrib[0]) return PTG_OK;
     int rc;
     const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
+    // d_part: 2 iterations x (gather partials, combine partials) + base + D0
+    const uint64_t part_floats = 2 * (2ull * g->grid + 2ull * kMaxCombineBlocks) + 4;
     if ((rc = galloc(g, (void **)&g->d_contrib[0], sizeof(float) * g->V)) ||
         (rc = galloc(g, (void **)&g->d_contrib[1], sizeof(float) * g->V)) ||
-        (rc = galloc(g, (void **)&g->d_dang, sizeof(float) * 4)) ||
-        (rc = galloc(g, (void **)&g->d_part, sizeof(float) * 2 * g->grid)) ||
+        (rc = galloc(g, (void **)&g->d_part, sizeof(float) * part_floats)) ||
         (rc = galloc(g, (void **)&g->d_lpart, sizeof(float) * 2 * (chunks + 1))))
         return rc;
     g->lchunks_cap = (uint32_t)chunks;
@@ -269,10 +298,16 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
     } else {
         const uint32_t blocks = std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid);
         k_pr_init<<<blocks, kBlock, 0, g->stream>>>(V, g->out.off, r0, g->d_contrib[0]);
+        // partial buffers: [it parity][gather | combine], then base, then D0
+        const uint32_t gb = g->grid, cb = combine_blocks(g);
+        float *part[2] = {g->d_part, g->d_part + 2 * gb};
+        float *cpart[2] = {g->d_part + 4 * gb, g->d_part + 4 * gb + 2 * kMaxCombineBlocks};
+        float *base = g->d_part + 4 * gb + 4 * kMaxCombineBlocks;
+        float *d0 = base + 2;
         // D0 = (number of dangling vertices) / V, split into (hi, lo).
-        const double d0 = (double)g->out.bins.n_zero / (double)V;
-        const float d0h = (float)d0;
-        k_set_f32x2<<<1, 64, 0, g->stream>>>(g->d_dang, d0h, (float)(d0 - (double)d0h));
+        const double d0v = (double)g->out.bins.n_zero / (double)V;
+        const float d0h = (float)d0v;
+        k_set_f32x2<<<1, 64, 0, g->stream>>>(d0, d0h, (float)(d0v - (double)d0h));
         PTG_HIP(hipGetLastError());
         for (int t = 0; t < iters; t++) {
             PrArgs a;
@@ -285,15 +320,19 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
             a.cin = g->d_contrib[t & 1];
             a.cout = g->d_contrib[(t + 1) & 1];
             a.rank_out = (t == iters - 1) ? ranks : nullptr;
-            a.dang_in = g->d_dang + 2 * (t & 1);
-            a.dang_out = g->d_dang + 2 * ((t + 1) & 1);
-            a.part = g->d_part;
+            a.d0 = t == 0 ? d0 : nullptr;
+            a.part_in = part[(t + 1) & 1];
+            a.cpart_in = cpart[(t + 1) & 1];
+            a.part_out = part[t & 1];
+            a.cpart_out = cpart[t & 1];
+            a.base_out = base;
             a.lpart = g->d_lpart;
-            a.gather_blocks = g->grid;
+            a.gather_blocks = gb;
+            a.combine_blocks = cb;
             a.damping = damping;
             a.inv_v = r0;
             timer.launch(k_pr_gather, dim3(g->grid), dim3(kBlock), a);
-            k_pr_combine<<<1, kBlock, 0, g->stream>>>(a);
+            k_pr_combine<<<cb, kBlock, 0, g->stream>>>(a);
             PTG_HIP(hipGetLastError());
         }
     }

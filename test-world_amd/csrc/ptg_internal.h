@@ -141,12 +141,11 @@ struct ptg_graph {
     uint32_t *d_qmem = nullptr;
     ptg::RoundCtr *d_ring = nullptr;    // kRing per-round counter blocks
     ptg::RoundCtr *h_ring = nullptr;    // pinned mirror
-    unsigned long long *d_acc = nullptr; // 4 accumulators for statistics kernels
+    unsigned long long *d_acc = nullptr; // 2 x kShards accumulators for statistics kernels
 
     // PageRank state.
     float *d_contrib[2] = {};
-    float *d_dang = nullptr;            // 2 x (hi, lo)
-    float *d_part = nullptr;            // per-workgroup dangling partials (hi, lo)
+    float *d_part = nullptr;            // PageRank dangling partials, base value, D0
     float *d_lpart = nullptr;           // hub chunk partials (hi, lo)
     uint32_t lchunks_cap = 0;
 

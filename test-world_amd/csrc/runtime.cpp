@@ -184,7 +184,6 @@ static void destroy(ptg_graph *g) {
     gfree(g, g->d_ring);
     gfree(g, g->d_acc);
     for (auto *c : g->d_contrib) gfree(g, c);
-    gfree(g, g->d_dang);
     gfree(g, g->d_part);
     gfree(g, g->d_lpart);
     if (g->h_ring) (void)hipHostFree(g->h_ring);
@@ -320,7 +319,7 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
         g->d_cb[i] = p + qwords + cwords;
     }
     TRY(galloc(g, (void **)&g->d_ring, sizeof(RoundCtr) * kRing));
-    TRY(galloc(g, (void **)&g->d_acc, sizeof(unsigned long long) * 4));
+    TRY(galloc(g, (void **)&g->d_acc, sizeof(unsigned long long) * 2 * kShards));
     if (hipHostMalloc((void **)&g->h_ring, sizeof(RoundCtr) * kRing, hipHostMallocDefault) != hipSuccess)
         TRY(fail(PTG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters)"));
     {
@@ -485,7 +484,7 @@ uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const pt
     b += al(4 * (uint64_t)V) + 5 * al(4 * nwords) + 3 * al(32 * nwords); // values, bitmaps, byte flags
     if (!(c.flags & PTG_FLAG_NO_RELABEL)) b += 3 * al(4 * (uint64_t)V); // permutation, inverse, results
     b += al(4 * 2 * ((uint64_t)kShards * kQueueCap + 2ull * kShards * chunk_queue_cap(V, grid)));  // queues
-    b += al(sizeof(RoundCtr) * kRing) + al(32);                        // counter ring, accumulators
+    b += al(sizeof(RoundCtr) * kRing) + al(16 * kShards);              // counter ring, accumulators
     return b;
 }
 

@@ -12,8 +12,10 @@ value = GTEPS of the traversal queries: sum over the 16 BFS/SSSP queries of
 m_r (sum of out-degrees of the reached vertices, directed edges) divided by
 their wall time. PageRank iterations/s and CC time are reported beside it.
 
-roofline: the query kind with the largest summed main-kernel time is the
-dominant kernel; its algorithmic bytes (BFS 12*V_r + 8*m_r, SSSP
+roofline: the query kind with the largest summed main-kernel time in the
+warmup steps is the dominant kernel, and only its launches are timed (HIP
+events around each launch on the graph's stream) in the timed steps; its
+algorithmic bytes (BFS 12*V_r + 8*m_r, SSSP
 12*V_r + 12*m_r, CC 16*V + 8*E, PageRank 8*E + 20*V per launch) divided by
 its HIP-event-timed launch durations give `achieved`.
 
@@ -32,6 +34,24 @@ sys.path.insert(0, os.path.join(ROOT, "test-world_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat22.md")
+
+
+def measured_traffic(kernel, scale):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary (two rocprofv3 --pmc passes scaled by a copy kernel), or None when
+    that summary does not cover this kernel and workload."""
+    if scale != 22 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    meta = {}
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            if ":" in line and line.startswith("traffic_"):
+                k, v = line.split(":", 1)
+                meta[k.strip()] = v.strip()
+    if meta.get("traffic_kernel") != kernel:
+        return None
+    return int(meta["traffic_bytes_per_launch"])
 
 
 def host_cores():
@@ -57,6 +77,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not time main-kernel launches with HIP events (no roofline)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB 3 times (hardware-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -67,6 +89,17 @@ def main():
     import ptgraph as pg
 
     pg.init(0)
+    if args.copy_probe:
+        # the library's streaming copy kernel (k_copy_bytes) reads and writes
+        # exactly MIB MiB each time, so its byte counters calibrate those of
+        # the graph kernels in the same profiled run
+        nb = args.copy_probe << 20
+        a, b = pg.DeviceBuffer(nb), pg.DeviceBuffer(nb)
+        for _ in range(3):
+            b.copy_from(a)
+        a.free()
+        b.free()
+        log("copy probe: 3 x %d MiB" % args.copy_probe)
     cores = host_cores()
     t0 = time.time()
     csr = pg.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, threads=cores)
@@ -76,39 +109,53 @@ def main():
     srcs = [int(s) for s in rng.choice(np.nonzero(deg > 0)[0], args.sources, replace=False)]
     log("graph V=%d E=%d generated in %.1fs; sources %s" % (csr.V, csr.E, time.time() - t0, srcs))
 
-    cfg = pg.default_config(flags=0 if args.no_timing else pg.FLAG_TIMING)
+    cfg = pg.default_config(flags=0)
     g = pg.Graph.from_csr(csr, config=cfg)
     V, E = csr.V, csr.E
     out = pg.DeviceBuffer(4 * V)
 
     kinds = ("bfs", "sssp", "cc", "pagerank")
 
-    def step(acc):
+    def set_timing(kind, timed):
+        on = not args.no_timing and (timed == "all" or kind == timed)
+        g.set_flags(pg.FLAG_TIMING if on else 0)
+
+    def step(acc, timed):
+        set_timing("bfs", timed)
         for s in srcs:
             t = time.perf_counter()
             _, st = g.bfs(s, out=out)
             acc["bfs"].append((time.perf_counter() - t, st.edges_traversed, st.reached, st.main_kernel_ns,
                                st.main_launches, st.iterations))
+        set_timing("sssp", timed)
         for s in srcs:
             t = time.perf_counter()
             _, st = g.sssp(s, out=out)
             acc["sssp"].append((time.perf_counter() - t, st.edges_traversed, st.reached, st.main_kernel_ns,
                                 st.main_launches, st.iterations))
+        set_timing("cc", timed)
         t = time.perf_counter()
         _, st = g.cc(out=out)
         acc["cc"].append((time.perf_counter() - t, E, V, st.main_kernel_ns, st.main_launches, st.iterations))
+        set_timing("pagerank", timed)
         t = time.perf_counter()
         _, st = g.pagerank(0.85, args.pr_iters, out=out)
         acc["pagerank"].append((time.perf_counter() - t, E, V, st.main_kernel_ns, st.main_launches,
                                 st.iterations))
 
+    # Warmup steps time every kind's main kernel to find the dominant one; the
+    # timed steps then time only that kernel, so the per-launch events do not
+    # slow the other queries.
+    wacc = {k: [] for k in kinds}
     for i in range(args.warmup):
-        step({k: [] for k in kinds})
+        step(wacc, "all")
         log("warmup step %d done" % (i + 1))
+    dom = max(kinds, key=lambda k: sum(r[3] for r in wacc[k])) if wacc["bfs"] else "sssp"
+    log("dominant kernel kind: %s" % dom)
     acc = {k: [] for k in kinds}
     t_start = time.perf_counter()
     for i in range(args.steps):
-        step(acc)
+        step(acc, dom)
     t_total = time.perf_counter() - t_start
     log("timed %d steps in %.3fs" % (args.steps, t_total))
 
@@ -134,19 +181,23 @@ def main():
             return 16 * V + 8 * E
         return args.pr_iters * (8 * E + 20 * V)
 
-    per_kind = {}
-    for k in kinds:
-        ns = tot(k, 3)
-        launches = tot(k, 4)
-        b = sum(algo_bytes(k, r) for r in acc[k])
-        per_kind[k] = {
-            "main_kernel_ms": ns / 1e6,
-            "launches": launches,
-            "algo_bytes": b,
-            "achieved_gbs": b / ns if ns else 0.0,
-            "avg_launch_us": ns / launches / 1e3 if launches else 0.0,
-        }
-    dom = max(kinds, key=lambda k: per_kind[k]["main_kernel_ms"])
+    def kind_stats(accd):
+        res = {}
+        for k in kinds:
+            ns = sum(r[3] for r in accd[k])
+            launches = sum(r[4] for r in accd[k])
+            b = sum(algo_bytes(k, r) for r in accd[k])
+            res[k] = {
+                "main_kernel_ms": ns / 1e6,
+                "launches": launches,
+                "algo_bytes": b,
+                "achieved_gbs": b / ns if ns else 0.0,
+                "avg_launch_us": ns / launches / 1e3 if launches else 0.0,
+            }
+        return res
+
+    per_kind = kind_stats(acc)
+    per_kind_warm = kind_stats(wacc)
     kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true>", "cc": "k_expand<Hook>",
                     "pagerank": "k_pr_gather"}
     achieved = per_kind[dom]["achieved_gbs"]
@@ -156,7 +207,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(kernel_names[dom], args.scale),
         "kernel": kernel_names[dom],
         "bytes_per_launch": int(per_kind[dom]["algo_bytes"] / max(1, per_kind[dom]["launches"])),
         "avg_launch_us": round(per_kind[dom]["avg_launch_us"], 2),
@@ -214,8 +265,9 @@ def main():
         "sssp_gteps": round(sssp_gteps, 4),
         "pagerank_iters_per_s": round(pr_iters_s, 2),
         "cc_ms": round(cc_ms, 3),
-        "per_kernel": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in d.items()}
-                       for k, d in per_kind.items()},
+        # every kind's main kernel, timed in the warmup steps
+        "per_kernel_warmup": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in d.items()}
+                              for k, d in per_kind_warm.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -118,6 +118,11 @@ PTG_API int ptg_device_alloc(uint64_t bytes, void **ptr);
 PTG_API int ptg_device_free(void *ptr);
 PTG_API int ptg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
 PTG_API int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+/* Device-to-device copy by the library's own streaming kernel (k_copy_bytes,
+ * 16 B per lane when both pointers are 16-byte aligned); synchronous. Its
+ * byte counters read exactly 2 x bytes of algorithmic traffic, which is what
+ * calibrates hardware counters. */
+PTG_API int ptg_memcpy_d2d(void *dst, const void *src, uint64_t bytes);
 
 /* ---- graph ---------------------------------------------------------------- */
 /* Checks the reference-layout CSR on the host: offsets non-decreasing,
@@ -142,6 +147,10 @@ PTG_API void ptg_graph_destroy(ptg_graph *g);
 PTG_API uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const ptg_config *cfg);
 PTG_API uint64_t ptg_graph_device_bytes(const ptg_graph *g);
 PTG_API int ptg_graph_is_symmetric(ptg_graph *g); /* 1, 0, or negative error */
+/* Changes the graph's runtime flags. Only PTG_FLAG_TIMING may differ from the
+ * flags the graph was created with (PTG_ERR_INVALID_ARG otherwise). No
+ * reference counterpart: the reference times whole calls with the host clock. */
+PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
 
 /* ---- queries ---------------------------------------------------------------- */
 /* bfs_kernel / bfsCaculateIn* (gpu_kernels.cu:118-134, main.cu:1041-1123,

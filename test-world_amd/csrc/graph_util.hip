@@ -28,6 +28,28 @@ int fill_u32(ptg_graph *g, uint32_t *p, uint64_t n, uint32_t v) {
     return PTG_OK;
 }
 
+// Streaming copy: uint4 per lane over the aligned body, bytes for the rest.
+__global__ __launch_bounds__(kBlock) void k_copy_bytes(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src,
+                                                       uint64_t n, bool vec) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n16 = vec ? n / 16 : 0;
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    for (uint64_t j = i; j < n16; j += stride) d4[j] = s4[j];
+    for (uint64_t j = n16 * 16 + i; j < n; j += stride) dst[j] = src[j];
+}
+
+int copy_bytes(void *dst, const void *src, uint64_t n, hipStream_t stream) {
+    if (n == 0) return PTG_OK;
+    const bool vec = (((uintptr_t)dst | (uintptr_t)src) & 15u) == 0;
+    const uint64_t units = vec ? n / 16 : n;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((units + kBlock - 1) / kBlock + 1, 4096);
+    k_copy_bytes<<<blocks, kBlock, 0, stream>>>(static_cast<uint8_t *>(dst), static_cast<const uint8_t *>(src), n, vec);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
 // out[v] = src[perm[v]]: a result in device ids back to the caller's ids.
 __global__ __launch_bounds__(kBlock) void k_unpermute(uint32_t V, const uint32_t *__restrict__ perm,
                                                       const uint32_t *__restrict__ src, uint32_t *__restrict__ dst) {

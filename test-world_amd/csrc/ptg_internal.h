@@ -181,6 +181,7 @@ void free_static_bins(ptg_graph *g, StaticBins *sb);
 
 // Kernel drivers (frontier.hip, pagerank.hip). `out` is a device pointer.
 int fill_u32(ptg_graph *g, uint32_t *p, uint64_t n, uint32_t v);
+int copy_bytes(void *dst, const void *src, uint64_t n, hipStream_t stream);
 int run_bfs(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_cc(ptg_graph *g, uint32_t *out, ptg_stats *st);

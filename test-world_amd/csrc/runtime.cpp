@@ -451,6 +451,17 @@ int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
     return PTG_OK;
 }
 
+int ptg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
+    if (bytes == 0) return PTG_OK;
+    if (!dst || !src) return fail(PTG_ERR_INVALID_ARG, "null pointer");
+    if (!is_device_pointer(dst) || !is_device_pointer(src))
+        return fail(PTG_ERR_INVALID_ARG, "ptg_memcpy_d2d needs two device pointers");
+    int rc = copy_bytes(dst, src, bytes, nullptr);
+    if (rc) return rc;
+    PTG_HIP(hipDeviceSynchronize());
+    return PTG_OK;
+}
+
 int ptg_graph_create(uint32_t V, uint64_t E, const uint32_t *nodePointers, const uint32_t *edgeList,
                      const EdgeWithWeight *wedges, const ptg_config *cfg, ptg_graph **out) {
     if (edgeList && wedges)
@@ -489,6 +500,14 @@ uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const pt
 }
 
 uint64_t ptg_graph_device_bytes(const ptg_graph *g) { return g ? g->device_bytes : 0; }
+
+int ptg_graph_set_flags(ptg_graph *g, uint32_t flags) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if ((flags ^ g->cfg.flags) & ~PTG_FLAG_TIMING)
+        return fail(PTG_ERR_INVALID_ARG, "only PTG_FLAG_TIMING can change after graph creation");
+    g->cfg.flags = flags;
+    return PTG_OK;
+}
 
 int ptg_graph_is_symmetric(ptg_graph *g) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");

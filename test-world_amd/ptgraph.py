@@ -111,6 +111,7 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_uint64)]),
     "ptg_device_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "ptg_device_free": (ctypes.c_int, [_vp]),
+    "ptg_memcpy_d2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "ptg_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "ptg_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "ptg_validate_csr": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p,
@@ -125,6 +126,7 @@ SIGNATURES = {
                                                    ctypes.POINTER(Config)]),
     "ptg_graph_device_bytes": (ctypes.c_uint64, [_vp]),
     "ptg_graph_is_symmetric": (ctypes.c_int, [_vp]),
+    "ptg_graph_set_flags": (ctypes.c_int, [_vp, ctypes.c_uint32]),
     "ptg_bfs": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
     "ptg_sssp": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
     "ptg_cc": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Stats)]),
@@ -306,6 +308,16 @@ class DeviceBuffer:
         _check(lib().ptg_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "ptg_memcpy_d2h")
         return out
 
+    def from_host(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().ptg_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes), "ptg_memcpy_h2d")
+
+    def copy_from(self, other, nbytes=None):
+        n = min(self.nbytes, other.nbytes) if nbytes is None else nbytes
+        _check(lib().ptg_memcpy_d2d(self.ptr, other.ptr, n), "ptg_memcpy_d2d")
+
     def free(self):
         if self.ptr:
             lib().ptg_device_free(self.ptr)
@@ -397,6 +409,10 @@ class Graph:
         if rc < 0:
             raise PtgError(rc, "ptg_graph_is_symmetric")
         return bool(rc)
+
+    def set_flags(self, flags):
+        """Changes runtime flags (only FLAG_TIMING may change after creation)."""
+        _check(lib().ptg_graph_set_flags(self._h, flags), "ptg_graph_set_flags")
 
     @property
     def device_bytes(self):

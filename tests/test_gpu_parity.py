@@ -245,3 +245,46 @@ def test_reference_named_drivers(rmat16):
     bad = ptr.copy()
     bad[3] = bad[4] + 1
     assert L.bfsCaculateInShare(g.V, g.E, pg._u32(bad), pg._u32(col), 0) == pg.ERR_INVALID_GRAPH
+
+
+def test_memcpy_d2d():
+    say("case memcpy d2d")
+    rng = np.random.default_rng(5)
+    n = (1 << 20) + 37
+    src = rng.integers(0, 256, n, dtype=np.uint8)
+    a, b = pg.DeviceBuffer(n), pg.DeviceBuffer(n)
+    try:
+        a.from_host(src)
+        for nb in (0, 1, 15, 16, 4097, n):
+            b.from_host(np.zeros(n, np.uint8))
+            b.copy_from(a, nb)
+            got = b.to_host(np.uint8, n)
+            np.testing.assert_array_equal(got[:nb], src[:nb])
+            assert not got[nb:].any()
+        # unaligned pointers take the byte path
+        b.from_host(np.zeros(n, np.uint8))
+        assert pg.lib().ptg_memcpy_d2d(b.ptr + 1, a.ptr + 3, n - 3) == pg.OK
+        got = b.to_host(np.uint8, n)
+        np.testing.assert_array_equal(got[1:n - 2], src[3:])
+        assert got[0] == 0 and not got[n - 2:].any()
+    finally:
+        a.free()
+        b.free()
+    host = np.zeros(16, np.uint8)
+    assert pg.lib().ptg_memcpy_d2d(host.ctypes.data, host.ctypes.data, 16) == pg.ERR_INVALID_ARG
+
+
+def test_set_flags_timing(rmat16):
+    g, gg = rmat16
+    say("case set_flags")
+    gg.set_flags(pg.FLAG_TIMING)
+    _, st = gg.bfs(0)
+    assert st.main_launches > 0 and st.main_kernel_ns > 0
+    gg.set_flags(0)
+    lv, st = gg.bfs(0)
+    assert st.main_launches == 0
+    want, _, _ = oracle.ref_loop(oracle.BFS, g.node_pointers, g.cols, src=0, threads=8)
+    np.testing.assert_array_equal(lv, want)
+    with pytest.raises(pg.PtgError) as e:
+        gg.set_flags(pg.FLAG_NO_RELABEL)
+    assert e.value.code == pg.ERR_INVALID_ARG

@@ -159,6 +159,18 @@ def main():
     t_total = time.perf_counter() - t_start
     log("timed %d steps in %.3fs" % (args.steps, t_total))
 
+    # One more pass of the traversal queries with host-memory outputs: the
+    # rate a caller sees when the result crosses PCIe (reported beside value).
+    g.set_flags(0)
+    pcie = {}
+    for kind, fn in (("bfs", g.bfs), ("sssp", g.sssp)):
+        t = time.perf_counter()
+        m = 0
+        for s in srcs:
+            _, st = fn(s)
+            m += st.edges_traversed
+        pcie[kind + "_gteps"] = round(m / (time.perf_counter() - t) / 1e9, 4)
+
     def tot(k, idx):
         return sum(r[idx] for r in acc[k])
 
@@ -268,6 +280,7 @@ def main():
         # every kind's main kernel, timed in the warmup steps
         "per_kernel_warmup": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in d.items()}
                               for k, d in per_kind_warm.items()},
+        "host_output_gteps": pcie,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

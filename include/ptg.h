@@ -76,6 +76,10 @@ typedef struct ptg_config {
     float bu_to_td;               /* BFS returns to top-down when n_f < V / bu_to_td */
     uint32_t sssp_delta;          /* near/far split step for SSSP; 0 = auto */
     uint32_t flags;               /* PTG_FLAG_* */
+    uint32_t batch_delta;         /* near/far step of batched SSSP; 0 = auto */
+    uint32_t batch_grid;          /* workgroups of the batched kernels; 0 = grid_blocks */
+    float batch_td_to_bu;         /* batched BFS goes bottom-up when m_f > m_u / this; 0 = td_to_bu */
+    uint32_t batch_chunk;         /* edges per static chunk of batched SSSP; 0 = auto (256) */
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
@@ -151,6 +155,12 @@ PTG_API int ptg_graph_is_symmetric(ptg_graph *g); /* 1, 0, or negative error */
  * flags the graph was created with (PTG_ERR_INVALID_ARG otherwise). No
  * reference counterpart: the reference times whole calls with the host clock. */
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
+/* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta, batch_delta,
+ * batch_grid, batch_td_to_bu, batch_chunk and PTG_FLAG_TIMING. Every other field must equal
+ * the graph's (PTG_ERR_INVALID_ARG otherwise): they shape allocations made at
+ * creation. Results never depend on these values, only speed does. */
+PTG_API int ptg_graph_set_config(ptg_graph *g, const ptg_config *cfg);
+PTG_API int ptg_graph_get_config(const ptg_graph *g, ptg_config *cfg);
 
 /* ---- queries ---------------------------------------------------------------- */
 /* bfs_kernel / bfsCaculateIn* (gpu_kernels.cu:118-134, main.cu:1041-1123,
@@ -166,6 +176,29 @@ PTG_API int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st);
  * D = rank mass of vertices with outdeg 0; FP32 with compensated sums. */
 PTG_API int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out,
                  ptg_stats *st);
+
+/* ---- batched multi-source queries ------------------------------------------ */
+/* Up to 64 sources in one batch (no reference counterpart: the reference runs
+ * one source per driver call, ptGraph/main.cu:537-556). Row k of the output
+ * (n x V words, source-major) equals ptg_bfs / ptg_sssp from srcs[k]: same
+ * values, same sentinels. The output may be host or device memory. Stats:
+ * reached = reached (source, vertex) pairs, edges_traversed = sum over the
+ * sources of m_r, iterations = levels / rounds run. */
+PTG_API int ptg_bfs_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *levels_out, ptg_stats *st);
+PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *dist_out, ptg_stats *st);
+
+/* ---- autotuner ------------------------------------------------------------- */
+/* Times the graph's own queries from `srcs` (n >= 1; the first 64 form the
+ * batch) over a small grid of tuning values, one knob at a time, keeps the
+ * fastest device time for each and applies the result to the graph (also
+ * returned in *best). `what` is a mask of PTG_TUNE_*. Results never depend
+ * on the tuning, only speed does. No reference counterpart (the reference
+ * hard-codes 56 x 1024 threads and its partition fraction, main.cu:318-319). */
+#define PTG_TUNE_BFS 0x1u         /* td_to_bu, bu_to_td */
+#define PTG_TUNE_SSSP 0x2u        /* sssp_delta */
+#define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
+#define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_grid, batch_chunk */
+PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);
 
 /* ---- C-ABI entry points named after the reference drivers ------------------ */
 /* Same signature and meaning as ptGraph/main.cu:24-56: build the graph, run one

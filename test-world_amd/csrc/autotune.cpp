@@ -1,0 +1,126 @@
+// Autotuner: picks the graph's runtime tuning (BFS push/pull switch
+// thresholds, SSSP near/far steps, launch width and static chunk size of the
+// batched kernels) by timing the graph's own queries.
+//
+// Coordinate search: each knob in turn is set to every value of its grid with
+// the others held, the query that knob affects is timed twice per value (HIP
+// events around the whole query, the faster run kept, so a one-time capture or
+// chunk-list rebuild is not charged), and the fastest value stays. Results do
+// not depend on any knob (every schedule reaches the same fixed point), so
+// only device time is compared.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ptg_internal.h"
+
+namespace ptg {
+
+namespace {
+
+enum Query { kBfs, kSssp, kBfsBatch, kSsspBatch };
+
+struct Tuner {
+    ptg_graph *g;
+    std::vector<uint32_t> plain;  // device ids, up to 4
+    std::vector<uint32_t> batch;  // caller ids, up to 64
+
+    int time_once(Query q, double *ns) {
+        ptg_stats st;
+        int rc = PTG_OK;
+        double t = 0;
+        if (q == kBfs || q == kSssp) {
+            for (uint32_t s : plain) {
+                rc = q == kBfs ? run_bfs(g, s, g->d_val, &st) : run_sssp(g, s, g->d_val, &st);
+                if (rc) return rc;
+                t += (double)st.kernel_ns;
+            }
+        } else {
+            uint32_t *out = nullptr;
+            if ((rc = batch_staging(g, &out))) return rc;
+            rc = run_batch_query(g, q == kSsspBatch, batch.data(), (uint32_t)batch.size(), out, &st);
+            if (rc) return rc;
+            t = (double)st.kernel_ns;
+        }
+        *ns = t;
+        return PTG_OK;
+    }
+
+    int time_best(Query q, double *ns) {
+        double a = 0, b = 0;
+        int rc = time_once(q, &a);
+        if (!rc) rc = time_once(q, &b);
+        *ns = std::min(a, b);
+        return rc;
+    }
+
+    // Tries every value of one knob; leaves the fastest one set.
+    template <class T>
+    int knob(Query q, T *field, const std::vector<T> &values) {
+        T best = *field;
+        double best_ns = 0;
+        int rc = time_best(q, &best_ns);
+        if (rc) return rc;
+        for (T v : values) {
+            if (v == best) continue;
+            const T keep = *field;
+            *field = v;
+            double ns = 0;
+            if ((rc = time_best(q, &ns))) {
+                *field = keep;
+                return rc;
+            }
+            if (ns < best_ns) {
+                best_ns = ns;
+                best = v;
+            }
+            *field = keep;
+        }
+        *field = best;
+        return PTG_OK;
+    }
+};
+
+}  // namespace
+
+}  // namespace ptg
+
+using namespace ptg;
+
+extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best) {
+    if (!g || !srcs || n < 1) return fail(PTG_ERR_INVALID_ARG, "autotune needs a graph and at least one source");
+    for (int i = 0; i < n; i++)
+        if (srcs[i] >= g->V) return fail(PTG_ERR_INVALID_ARG, "source out of range");
+    ptg_config &c = g->cfg;
+    // explicit values for every knob the search moves
+    if (c.td_to_bu <= 0) c.td_to_bu = 14.0f;
+    if (c.bu_to_td <= 0) c.bu_to_td = 24.0f;
+    if (!c.sssp_delta) c.sssp_delta = 16;
+    if (c.batch_td_to_bu <= 0) c.batch_td_to_bu = c.td_to_bu;
+    if (!c.batch_grid) c.batch_grid = 2 * g->grid;
+    if (!c.batch_delta) c.batch_delta = 256;
+    if (!c.batch_chunk) c.batch_chunk = 256;
+    Tuner t{g, {}, {}};
+    for (int i = 0; i < n && i < 4; i++) t.plain.push_back(g->relabeled ? g->h_perm[srcs[i]] : srcs[i]);
+    for (int i = 0; i < n && i < 64; i++) t.batch.push_back(srcs[i]);
+    const uint32_t G = g->grid;
+    int rc = PTG_OK;
+    if (what & PTG_TUNE_BFS) {
+        if (!rc) rc = t.knob(kBfs, &c.td_to_bu, std::vector<float>{4, 8, 14, 24, 40});
+        if (!rc) rc = t.knob(kBfs, &c.bu_to_td, std::vector<float>{8, 16, 24, 48});
+    }
+    if ((what & PTG_TUNE_SSSP) && g->weighted && !rc)
+        rc = t.knob(kSssp, &c.sssp_delta, std::vector<uint32_t>{8, 12, 16, 24, 32, 48});
+    if (what & PTG_TUNE_BFS_BATCH) {
+        if (!rc) rc = t.knob(kBfsBatch, &c.batch_td_to_bu, std::vector<float>{2, 6, 14, 30});
+        if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G});
+    }
+    if ((what & PTG_TUNE_SSSP_BATCH) && g->weighted) {
+        if (!rc) rc = t.knob(kSsspBatch, &c.batch_delta, std::vector<uint32_t>{32, 64, 128, 256, 1024});
+        if (!rc) rc = t.knob(kSsspBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G});
+        if (!rc) rc = t.knob(kSsspBatch, &c.batch_chunk, std::vector<uint32_t>{64, 128, 256, 512, 1024});
+    }
+    if (best) *best = c;
+    return rc;
+}

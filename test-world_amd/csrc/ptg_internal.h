@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "ptg.h"
@@ -108,6 +109,10 @@ struct Csr {
 
 }  // namespace ptg
 
+namespace ptg {
+struct BatchState;  // batch.hip
+}
+
 // The graph handle.
 struct ptg_graph {
     uint32_t V = 0;
@@ -160,8 +165,12 @@ struct ptg_graph {
     uint32_t *d_inv = nullptr;          // new id -> old id
     uint32_t *d_val2 = nullptr;         // V: results mapped back to old ids
 
+    // Batched multi-source state (batch.hip), allocated on the first batch.
+    ptg::BatchState *batch = nullptr;
+
     int symmetric = -1;  // -1 unknown, 0/1 once checked
-    uint64_t device_bytes = 0;
+    uint64_t device_bytes = 0;                          // charged against the budget
+    std::unordered_map<void *, uint64_t> alloc_bytes;   // live allocations and their sizes
     bool weighted = false;
 };
 
@@ -190,6 +199,11 @@ int check_symmetric(ptg_graph *g, int *sym);
 int ensure_in_edges(ptg_graph *g);
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,
                 uint64_t *edges);
+// Batched queries (batch.hip): `out` is a device buffer of n x V words in
+// the caller's ids; batch_staging gives a library-owned one of 64 x V.
+int run_batch_query(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, uint32_t *out, ptg_stats *st);
+int batch_staging(ptg_graph *g, uint32_t **out);
+void destroy_batch(ptg_graph *g);
 // dst[v] = src[perm[v]] for v < V (new-id results back to old ids).
 int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst);
 

@@ -49,11 +49,18 @@ int galloc(ptg_graph *g, void **p, uint64_t bytes) {
         return hip_fail(e, "hipMalloc");
     }
     g->device_bytes += bytes;
+    g->alloc_bytes[*p] = bytes;
     return PTG_OK;
 }
 
-void gfree(ptg_graph *, void *p) {
-    if (p) (void)hipFree(p);
+void gfree(ptg_graph *g, void *p) {
+    if (!p) return;
+    auto it = g->alloc_bytes.find(p);
+    if (it != g->alloc_bytes.end()) {
+        g->device_bytes -= it->second;
+        g->alloc_bytes.erase(it);
+    }
+    (void)hipFree(p);
 }
 
 uint32_t chunk_queue_cap(uint32_t V, uint32_t grid) {
@@ -184,6 +191,7 @@ static void destroy(ptg_graph *g) {
     gfree(g, g->d_ring);
     gfree(g, g->d_acc);
     for (auto *c : g->d_contrib) gfree(g, c);
+    destroy_batch(g);
     gfree(g, g->d_part);
     gfree(g, g->d_lpart);
     if (g->h_ring) (void)hipHostFree(g->h_ring);
@@ -501,6 +509,25 @@ uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const pt
 
 uint64_t ptg_graph_device_bytes(const ptg_graph *g) { return g ? g->device_bytes : 0; }
 
+int ptg_graph_set_config(ptg_graph *g, const ptg_config *c) {
+    if (!g || !c) return fail(PTG_ERR_INVALID_ARG, "null graph or config");
+    const ptg_config &o = g->cfg;
+    if (c->device_budget_bytes != o.device_budget_bytes || c->chunk_bytes != o.chunk_bytes ||
+        c->sssp_sentinel != o.sssp_sentinel || c->grid_blocks != o.grid_blocks || c->small_degree != o.small_degree ||
+        c->large_degree != o.large_degree || ((c->flags ^ o.flags) & ~PTG_FLAG_TIMING))
+        return fail(PTG_ERR_INVALID_ARG, "only tuning fields can change after graph creation");
+    if (!(c->td_to_bu >= 0) || !(c->bu_to_td >= 0) || !(c->batch_td_to_bu >= 0))
+        return fail(PTG_ERR_INVALID_ARG, "negative or NaN switch threshold");
+    g->cfg = *c;
+    return PTG_OK;
+}
+
+int ptg_graph_get_config(const ptg_graph *g, ptg_config *c) {
+    if (!g || !c) return fail(PTG_ERR_INVALID_ARG, "null graph or config");
+    *c = g->cfg;
+    return PTG_OK;
+}
+
 int ptg_graph_set_flags(ptg_graph *g, uint32_t flags) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if ((flags ^ g->cfg.flags) & ~PTG_FLAG_TIMING)
@@ -541,6 +568,34 @@ int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out, 
     if (iterations < 0 || !(damping >= 0.0f && damping <= 1.0f))
         return fail(PTG_ERR_INVALID_ARG, "pagerank: need iterations >= 0 and 0 <= damping <= 1");
     return with_output(g, ranks_out, [&](float *o) { return run_pagerank(g, damping, iterations, o, st); });
+}
+
+// Shared checks and output delivery of the two batched queries.
+static int batch_query(ptg_graph *g, bool sssp, const uint32_t *srcs, int n, uint32_t *out, ptg_stats *st) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (!srcs || !out) return fail(PTG_ERR_INVALID_ARG, "null sources or output");
+    if (n < 1 || n > 64) return fail(PTG_ERR_INVALID_ARG, "a batch holds 1 to 64 sources");
+    for (int i = 0; i < n; i++)
+        if (srcs[i] >= g->V) return fail(PTG_ERR_INVALID_ARG, "source out of range");
+    if (sssp && !g->weighted) return fail(PTG_ERR_UNSUPPORTED, "SSSP needs a weighted graph");
+    ptg_stats local;
+    if (!st) st = &local;
+    if (is_device_pointer(out)) return run_batch_query(g, sssp, srcs, (uint32_t)n, out, st);
+    uint32_t *stage = nullptr;
+    int rc = batch_staging(g, &stage);
+    if (rc) return rc;
+    if ((rc = run_batch_query(g, sssp, srcs, (uint32_t)n, stage, st))) return rc;
+    PTG_HIP(hipMemcpyAsync(out, stage, 4ull * g->V * (uint64_t)n, hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    return PTG_OK;
+}
+
+int ptg_bfs_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *levels_out, ptg_stats *st) {
+    return batch_query(g, false, srcs, n, levels_out, st);
+}
+
+int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *dist_out, ptg_stats *st) {
+    return batch_query(g, true, srcs, n, dist_out, st);
 }
 
 }  // extern "C"

@@ -6,11 +6,22 @@
 
 namespace ptg {
 
+// op.active(sv) when the op defines it (vertices it returns false for are
+// skipped), else true.
+template <class Op, class T>
+__device__ __forceinline__ auto op_active(const Op &op, T sv, int) -> decltype(op.active(sv)) {
+    return op.active(sv);
+}
+template <class Op, class T>
+__device__ __forceinline__ bool op_active(const Op &, T, long) {
+    return true;
+}
+
 // Walks the L (hub chunks of kChunkEdges, one workgroup each), M (one wave
 // per vertex) and S (one lane per vertex) bins and calls op.edge() from
-// wave-uniform control flow.
+// wave-uniform control flow. op.vertex(v) gives the value passed to edge().
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_expand(Frontier in, const uint32_t *__restrict__ off, Op op) {
+__device__ __forceinline__ void expand_static(const Frontier &in, const uint32_t *__restrict__ off, const Op &op) {
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
     const unsigned long long lp = in.ctr->l_pack;
@@ -25,7 +36,8 @@ __global__ __launch_bounds__(kBlock) void k_expand(Frontier in, const uint32_t *
         const uint32_t v = in.ql[lo];
         const uint64_t b = (uint64_t)off[v] + (uint64_t)(c - in.lb[lo]) * kChunkEdges;
         const uint64_t e = min((uint64_t)off[v + 1], b + kChunkEdges);
-        const uint32_t sv = op.vertex(v);
+        const auto sv = op.vertex(v);
+        if (!op_active(op, sv, 0)) continue;
         for (uint64_t base = b + wib * 64; base < e; base += kBlock) op.edge(base + lane < e, sv, base + lane);
     }
     const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
@@ -33,24 +45,33 @@ __global__ __launch_bounds__(kBlock) void k_expand(Frontier in, const uint32_t *
     const uint32_t nm = in.ctr->n_m;
     for (uint32_t i = gw; i < nm; i += nw) {
         const uint32_t v = in.qm[i];
+        const auto sv = op.vertex(v);
+        if (!op_active(op, sv, 0)) continue;
         const uint64_t b = off[v], e = off[v + 1];
-        const uint32_t sv = op.vertex(v);
         for (uint64_t base = b; base < e; base += 64) op.edge(base + lane < e, sv, base + lane);
     }
     const uint32_t ns = in.ctr->n_s;
     for (uint64_t base = (uint64_t)gw * 64; base < ns; base += (uint64_t)nw * 64) {
         const uint64_t i = base + lane;
         const bool valid = i < ns;
-        uint32_t b = 0, d = 0, sv = 0;
+        uint32_t b = 0, d = 0;
+        decltype(op.vertex(0u)) sv{};
         if (valid) {
             const uint32_t v = in.qs[i];
-            b = off[v];
-            d = off[v + 1] - b;
             sv = op.vertex(v);
+            if (op_active(op, sv, 0)) {
+                b = off[v];
+                d = off[v + 1] - b;
+            }
         }
         const uint32_t md = wave_max_u32(d);
         for (uint32_t j = 0; j < md; j++) op.edge(j < d, sv, (uint64_t)b + j);
     }
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_expand(Frontier in, const uint32_t *__restrict__ off, Op op) {
+    expand_static(in, off, op);
 }
 
 }  // namespace ptg

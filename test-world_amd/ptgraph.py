@@ -29,6 +29,7 @@ ERR_IO = -9
 FLAG_NO_VALIDATE = 0x1
 FLAG_TIMING = 0x2
 FLAG_NO_RELABEL = 0x4
+TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH = 0x1, 0x2, 0x4, 0x8
 
 BFS_UNREACHED = 0xFFFFFFFF  # ptGraph/main.cu:1068
 SSSP_SENTINEL_SHARE = 0xFFFFFFFE  # ptGraph/main.cu:584
@@ -66,6 +67,10 @@ class Config(ctypes.Structure):
         ("bu_to_td", ctypes.c_float),
         ("sssp_delta", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
+        ("batch_delta", ctypes.c_uint32),
+        ("batch_grid", ctypes.c_uint32),
+        ("batch_td_to_bu", ctypes.c_float),
+        ("batch_chunk", ctypes.c_uint32),
     ]
 
 
@@ -127,10 +132,16 @@ SIGNATURES = {
     "ptg_graph_device_bytes": (ctypes.c_uint64, [_vp]),
     "ptg_graph_is_symmetric": (ctypes.c_int, [_vp]),
     "ptg_graph_set_flags": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "ptg_graph_set_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
+    "ptg_graph_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
+    "ptg_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.POINTER(Config)]),
     "ptg_bfs": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
     "ptg_sssp": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
     "ptg_cc": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Stats)]),
     "ptg_pagerank": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
+    "ptg_bfs_batch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
+    "ptg_sssp_batch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "bfsCaculateInShare": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
     "bfsCaculateInOpt": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
     "bfsCaculateInAsync": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
@@ -404,6 +415,27 @@ class Graph:
     def pagerank(self, damping=0.85, iters=20, out=None):
         return self._run(lib().ptg_pagerank, (ctypes.c_float(damping), iters), out, np.float32, "ptg_pagerank")
 
+    def _run_batch(self, fn, srcs, out, name):
+        srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+        n = int(srcs.size)
+        st = Stats()
+        if out is None:
+            host = np.empty((n, self.V), dtype=np.uint32)
+            _check(fn(self._h, _u32(srcs), n, host.ctypes.data, ctypes.byref(st)), name)
+            return host, st
+        if out.nbytes < 4 * n * self.V:
+            raise ValueError("output buffer smaller than n x V words")
+        _check(fn(self._h, _u32(srcs), n, out.ptr, ctypes.byref(st)), name)
+        return out, st
+
+    def bfs_batch(self, srcs, out=None):
+        """Levels from up to 64 sources at once: row k == bfs(srcs[k])."""
+        return self._run_batch(lib().ptg_bfs_batch, srcs, out, "ptg_bfs_batch")
+
+    def sssp_batch(self, srcs, out=None):
+        """Distances from up to 64 sources at once: row k == sssp(srcs[k])."""
+        return self._run_batch(lib().ptg_sssp_batch, srcs, out, "ptg_sssp_batch")
+
     def is_symmetric(self):
         rc = lib().ptg_graph_is_symmetric(self._h)
         if rc < 0:
@@ -413,6 +445,25 @@ class Graph:
     def set_flags(self, flags):
         """Changes runtime flags (only FLAG_TIMING may change after creation)."""
         _check(lib().ptg_graph_set_flags(self._h, flags), "ptg_graph_set_flags")
+
+    def get_config(self):
+        c = Config()
+        _check(lib().ptg_graph_get_config(self._h, ctypes.byref(c)), "ptg_graph_get_config")
+        return c
+
+    def tune(self, **changes):
+        """Changes tuning fields (td_to_bu, bu_to_td, sssp_delta, batch_*)."""
+        c = self.get_config()
+        for k, v in changes.items():
+            setattr(c, k, v)
+        _check(lib().ptg_graph_set_config(self._h, ctypes.byref(c)), "ptg_graph_set_config")
+
+    def autotune(self, srcs, what=0xF):
+        """Tunes the graph on sample sources; returns the Config it applied."""
+        srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+        c = Config()
+        _check(lib().ptg_autotune(self._h, _u32(srcs), int(srcs.size), what, ctypes.byref(c)), "ptg_autotune")
+        return c
 
     @property
     def device_bytes(self):

@@ -200,6 +200,79 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
 #define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_grid, batch_chunk */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);
 
+/* ---- serving ----------------------------------------------------------------- */
+/* A request loop over resident graphs (no reference counterpart: the
+ * reference harness runs one query per process, ptGraph/main.cu:78-86).
+ * Each graph gets a worker thread on its own stream, so work on different
+ * graphs overlaps on the device. Continuous batching: whenever a graph's slot
+ * is free, its pending BFS (or SSSP) requests, up to max_batch, run as one
+ * batched launch; identical CC / PageRank requests on one graph share one
+ * run. Admission by device memory: the query state a launch needs is charged
+ * against state_budget_bytes; idle graphs' state is released
+ * (least recently used first) to make room, else the launch waits for running
+ * ones; requests that can never fit complete with PTG_ERR_REJECTED.
+ * Every result equals the single query's (same values; checksum below). */
+typedef struct ptg_server ptg_server;
+
+typedef struct ptg_server_config {
+    uint64_t state_budget_bytes; /* device bytes for lazily allocated query state; 0 = unlimited */
+    uint32_t max_batch;          /* sources per BFS/SSSP launch, 1..64; 1 = plain per-query path */
+    uint32_t max_inflight;       /* graphs executing at once; 0 = all */
+    uint32_t coalesce;           /* share one run among identical CC / PageRank requests */
+    uint32_t queue_cap;          /* pending requests per graph beyond which submit rejects; 0 = 1<<20 */
+    uint32_t min_batch;          /* fewer pending BFS/SSSP requests than this run one at a time; 0 = 8 */
+} ptg_server_config;
+
+enum { PTG_Q_BFS = 0, PTG_Q_SSSP = 1, PTG_Q_CC = 2, PTG_Q_PAGERANK = 3 };
+
+typedef struct ptg_request {
+    uint32_t graph;      /* id from ptg_server_add_graph */
+    uint32_t kind;       /* PTG_Q_* */
+    uint32_t source;     /* BFS / SSSP */
+    uint32_t iterations; /* PageRank */
+    float damping;       /* PageRank */
+    uint32_t pad;
+    void *out;           /* V words, host or device memory; NULL = keep only the checksum */
+    uint64_t tag;        /* returned in the completion */
+} ptg_request;
+
+typedef struct ptg_completion {
+    uint64_t tag;
+    int32_t status;          /* PTG_OK or a negative code */
+    uint32_t batch_size;     /* requests served by the same launch */
+    uint64_t checksum;       /* sum over v of out[v] * (2v + 1) mod 2^64 (result words, caller ids) */
+    uint64_t submit_ns;      /* steady clock: submitted, launch started, result delivered */
+    uint64_t start_ns;
+    uint64_t done_ns;
+} ptg_completion;
+
+typedef struct ptg_server_summary {
+    uint64_t completed, rejected, launches;
+    uint64_t p50_ns, p99_ns, max_ns; /* latency submit -> done over completed requests */
+    double mean_batch;               /* requests per launch */
+    uint64_t first_submit_ns, last_done_ns;
+    uint64_t state_bytes_peak;       /* most query-state bytes resident at once */
+    uint64_t evictions;              /* graphs whose state was released to admit another */
+} ptg_server_summary;
+
+PTG_API int ptg_server_create(const ptg_server_config *cfg, ptg_server **out);
+/* The graph must outlive the server and not be used elsewhere meanwhile. */
+PTG_API int ptg_server_add_graph(ptg_server *s, ptg_graph *g, uint32_t *id);
+PTG_API int ptg_server_submit(ptg_server *s, const ptg_request *r);
+/* Submits r[i] at first_call + arrival_ns[i] (steady clock, open loop) from
+ * a thread of its own and returns once all are submitted. */
+PTG_API int ptg_server_replay(ptg_server *s, const ptg_request *r, const uint64_t *arrival_ns, uint64_t n);
+/* Copies up to `max` completions; waits up to timeout_ms for the first.
+ * Returns the number copied or a negative code. */
+PTG_API int ptg_server_poll(ptg_server *s, ptg_completion *out, int max, int timeout_ms);
+PTG_API int ptg_server_drain(ptg_server *s); /* waits until every submitted request completed */
+/* The first failed launch's message ("" if none); valid until destroy. */
+PTG_API const char *ptg_server_error(ptg_server *s);
+/* paused != 0: no new launch starts (running ones finish; submits queue up). */
+PTG_API int ptg_server_pause(ptg_server *s, int paused);
+PTG_API int ptg_server_summary_get(ptg_server *s, ptg_server_summary *out);
+PTG_API void ptg_server_destroy(ptg_server *s);
+
 /* ---- C-ABI entry points named after the reference drivers ------------------ */
 /* Same signature and meaning as ptGraph/main.cu:24-56: build the graph, run one
  * query, free everything, return elapsed milliseconds (or a negative code). */

@@ -48,7 +48,8 @@ struct BCtr {
     unsigned long long mf[kShards];    // their out-degree sum
     unsigned long long munf[kShards];  // BFS: out-degree sum of vertices some source has not reached;
                                        // SSSP: edges visited by the round
-    unsigned long long far[kShards];   // SSSP: far masks left non-empty
+    unsigned long long far[kShards];   // SSSP: far masks left non-empty; BFS: OR of the frontier
+                                       // masks (sources that can still reach new vertices)
     uint32_t thr;                      // SSSP: threshold of the frontier this entry describes
     uint32_t overflow;                 // BFS: more than kLevelCap levels
     uint32_t pad[126];
@@ -177,6 +178,9 @@ __global__ void k_batch_advance(BParams *p, uint32_t k) {
 __device__ __forceinline__ bool bfs_bottom_up_dir(const BatchEnv &e, const BSum &s) {
     return e.symmetric && (double)s.mf * (double)e.alpha > (double)s.munf;
 }
+// Only sources with a non-empty frontier can reach anything new, so a
+// vertex looks only for their bits: the bits of sources that cannot reach it
+// would otherwise keep every bottom-up scan from stopping early.
 
 // Top-down: OR the frontier mask into each neighbour's next mask, only with
 // bits that neighbour has not seen.
@@ -218,6 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_finish(BatchEnv e, uint32_t j)
     if (s.nf == 0) return;
     const bool bu = bfs_bottom_up_dir(e, s);
     const unsigned long long mask = e.p->mask;
+    const unsigned long long live = s.far & mask;  // sources whose frontier is non-empty
     const unsigned long long *cur = rot(e, L);
     unsigned long long *nxt = rot(e, L + 1);
     unsigned long long *clr = rot(e, L + 2);
@@ -226,12 +231,12 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_finish(BatchEnv e, uint32_t j)
     const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     const uint32_t np = (e.nva + 63) / 64;
-    uint64_t t_nf = 0, t_mf = 0, t_munf = 0;
+    uint64_t t_nf = 0, t_mf = 0, t_munf = 0, t_live = 0;
     for (uint32_t p = gw; p < np; p += nw) {
         const uint32_t v = p * 64 + lane;
         const bool inb = v < e.nva;
         const unsigned long long sn = inb ? e.seen[v] : mask;
-        const unsigned long long unseen = ~sn & mask;
+        const unsigned long long unseen = ~sn & live;
         uint32_t b = 0, d = 0;
         if (inb) {
             b = e.off[v];
@@ -281,9 +286,10 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_finish(BatchEnv e, uint32_t j)
         }
         t_nf += nb != 0;
         t_mf += nb ? d : 0;
-        t_munf += (inb && (sn | nb) != mask) ? d : 0;
+        t_munf += (inb && (~(sn | nb) & live)) ? d : 0;
+        t_live |= nb;
     }
-    bctr_add(out, t_nf, t_mf, t_munf, 0);
+    bctr_add(out, t_nf, t_mf, t_munf, t_live);
 }
 
 // Seeds level 0 from the device copy of the sources (one workgroup).
@@ -314,6 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_seed(BatchEnv e, const uint32_
         e.ring[0].nf[0] = nf;
         e.ring[0].mf[0] = mf;
         e.ring[0].munf[0] = E;
+        e.ring[0].far[0] = n == 64 ? ~0ull : ((1ull << n) - 1);
         e.p->n = n;
         e.p->base = 0;
         e.p->mask = n == 64 ? ~0ull : ((1ull << n) - 1);
@@ -602,7 +609,7 @@ static int ensure_batch(ptg_graph *g, bool sssp) {
             gfree(g, b->ch_b);
             b->ch_v = b->ch_b = nullptr;
             std::vector<uint32_t> off(V + 1), cv, cb;
-            PTG_HIP(hipMemcpy(off.data(), g->out.off, 4 * (V + 1), hipMemcpyDeviceToHost));
+            if ((rc = copy_sync(g, off.data(), g->out.off, 4 * (V + 1), hipMemcpyDeviceToHost))) return rc;
             for (uint32_t v = 0; v < g->V; v++)
                 for (uint32_t x = off[v]; x < off[v + 1]; x += std::min(chunk, off[v + 1] - x)) {
                     cv.push_back(v);
@@ -612,8 +619,9 @@ static int ensure_batch(ptg_graph *g, bool sssp) {
             if ((rc = galloc(g, (void **)&b->ch_v, 4 * n)) || (rc = galloc(g, (void **)&b->ch_b, 4 * n))) return rc;
             b->chunk = chunk;
             if (!cv.empty()) {
-                PTG_HIP(hipMemcpy(b->ch_v, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice));
-                PTG_HIP(hipMemcpy(b->ch_b, cb.data(), 4 * cb.size(), hipMemcpyHostToDevice));
+                if ((rc = copy_sync(g, b->ch_v, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice)) ||
+                    (rc = copy_sync(g, b->ch_b, cb.data(), 4 * cb.size(), hipMemcpyHostToDevice)))
+                    return rc;
             }
             b->nch = (uint32_t)cv.size();
         }
@@ -656,7 +664,9 @@ static BatchEnv make_batch_env(ptg_graph *g, int sym) {
 template <class Body>
 static int capture(ptg_graph *g, hipGraphExec_t *exec, Body body) {
     hipGraph_t graph = nullptr;
-    PTG_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    // relaxed: other threads (serving workers on other graphs) keep
+    // allocating and copying while this stream records
+    PTG_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
     const int brc = body();
     hipError_t err = hipStreamEndCapture(g->stream, &graph);
     if (brc) {
@@ -827,6 +837,25 @@ int batch_staging(ptg_graph *g, uint32_t **out) {
     }
     *out = b->out;
     return PTG_OK;
+}
+
+// Device bytes the next batch of this kind would allocate (state not yet
+// resident), staging for n x V outputs included.
+uint64_t batch_state_need(const ptg_graph *g, bool sssp) {
+    const BatchState *b = g->batch;
+    const uint64_t V = g->V;
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    uint64_t need = 0;
+    if (!b || !b->ring) need += 4 * al(8 * V) + al(sizeof(BCtr) * kBRing) + 2 * al(256);
+    if (!b || !b->out) need += al(4 * V * kBatchMax);
+    if (!sssp && (!b || !b->lm)) need += al(8 * V * kLevelCap);
+    if (sssp) {
+        if (!b || !b->far) need += al(8 * V);
+        if (!b || !b->dist) need += al(4 * V * kBatchMax);
+        const uint32_t chunk = g->cfg.batch_chunk ? g->cfg.batch_chunk : kBatchChunk;
+        if (!b || !b->ch_v || b->chunk != chunk) need += 2 * al(4 * (V + g->E / chunk + 1));
+    }
+    return need;
 }
 
 void destroy_batch(ptg_graph *g) {

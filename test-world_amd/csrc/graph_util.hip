@@ -178,4 +178,55 @@ int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t 
     return PTG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Row checksums: sum over v of row[v] * (2v + 1) mod 2^64, per row.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void flush_row(unsigned long long x, unsigned long long *dst, unsigned long long *s) {
+    x = wave_sum_u64(x);
+    if (lane_id() == 0) s[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) a += s[w];
+        atomicAdd(dst, a);
+    }
+    __syncthreads();
+}
+
+// Workgroup b sums the contiguous tile range [b*T, (b+1)*T) of the n rows
+// and adds once per row it touched.
+__global__ __launch_bounds__(kBlock) void k_row_checksums(const uint32_t *__restrict__ rows, uint32_t n, uint32_t V,
+                                                          unsigned long long *out) {
+    __shared__ unsigned long long s[kWavesPerBlock];
+    const uint64_t per_row = ((uint64_t)V + kBlock - 1) / kBlock;
+    const uint64_t total = per_row * n;
+    const uint64_t T = (total + gridDim.x - 1) / gridDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * T, t1 = min(total, t0 + T);
+    if (t0 >= t1) return;
+    uint32_t cur = (uint32_t)(t0 / per_row);
+    unsigned long long acc = 0;
+    for (uint64_t t = t0; t < t1; t++) {
+        const uint32_t r = (uint32_t)(t / per_row);
+        if (r != cur) {
+            flush_row(acc, &out[cur], s);
+            acc = 0;
+            cur = r;
+        }
+        const uint64_t v = (t % per_row) * kBlock + threadIdx.x;
+        if (v < V) acc += (unsigned long long)rows[(uint64_t)r * V + v] * (2 * v + 1);
+    }
+    flush_row(acc, &out[cur], s);
+}
+
+int row_checksums(ptg_graph *g, hipStream_t stream, const uint32_t *rows, uint32_t n, unsigned long long *d_out) {
+    if (n == 0) return PTG_OK;
+    k_fill_u32<<<1, kBlock, 0, stream>>>(reinterpret_cast<uint32_t *>(d_out), 2ull * n, 0u, false);
+    const uint64_t tiles = ((uint64_t)g->V + kBlock - 1) / kBlock * n;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(tiles, 1024);
+    k_row_checksums<<<blocks, kBlock, 0, stream>>>(rows, n, g->V, d_out);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
 }  // namespace ptg

@@ -241,8 +241,9 @@ int ensure_in_edges(ptg_graph *g) {
     const uint32_t V = g->V;
     const uint64_t E = g->E;
     std::vector<uint32_t> off((size_t)V + 1), col(E ? E : 1), toff((size_t)V + 1, 0), tcol(E ? E : 1);
-    PTG_HIP(hipMemcpy(off.data(), g->out.off, sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyDeviceToHost));
-    if (E) PTG_HIP(hipMemcpy(col.data(), g->out.col, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+    if ((rc = copy_sync(g, off.data(), g->out.off, sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyDeviceToHost)) ||
+        (rc = copy_sync(g, col.data(), g->out.col, sizeof(uint32_t) * E, hipMemcpyDeviceToHost)))
+        return rc;
     for (uint64_t e = 0; e < E; e++) toff[col[e] + 1]++;
     for (uint32_t v = 0; v < V; v++) toff[v + 1] += toff[v];
     std::vector<uint32_t> cur(toff.begin(), toff.end() - 1);
@@ -251,8 +252,9 @@ int ensure_in_edges(ptg_graph *g) {
     if ((rc = galloc(g, (void **)&g->in.off, sizeof(uint32_t) * ((uint64_t)V + 1))) ||
         (rc = galloc(g, (void **)&g->in.col, sizeof(uint32_t) * E)))
         return rc;
-    PTG_HIP(hipMemcpy(g->in.off, toff.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice));
-    if (E) PTG_HIP(hipMemcpy(g->in.col, tcol.data(), sizeof(uint32_t) * E, hipMemcpyHostToDevice));
+    if ((rc = copy_sync(g, g->in.off, toff.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, g->in.col, tcol.data(), sizeof(uint32_t) * E, hipMemcpyHostToDevice)))
+        return rc;
     if ((rc = build_static_bins(g, toff.data(), V, &g->in.bins))) return rc;
     g->in_is_out = false;
     g->have_in = true;
@@ -279,6 +281,27 @@ static int ensure_pr_state(ptg_graph *g) {
         return rc;
     g->lchunks_cap = (uint32_t)chunks;
     return PTG_OK;
+}
+
+uint64_t pr_state_need(const ptg_graph *g) {
+    if (g->d_contrib[0]) return 0;
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
+    uint64_t need = 2 * al(4ull * g->V) + al(4 * (2 * (2ull * g->grid + 2ull * kMaxCombineBlocks) + 4)) +
+                    al(8 * (chunks + 1));
+    if (!g->have_in) need += al(4ull * (g->V + 1)) + al(4 * g->E) + al(8ull * g->V);  // upper bound: transpose
+    return need;
+}
+
+void release_pr_state(ptg_graph *g) {
+    for (auto *&c : g->d_contrib) {
+        gfree(g, c);
+        c = nullptr;
+    }
+    gfree(g, g->d_part);
+    gfree(g, g->d_lpart);
+    g->d_part = g->d_lpart = nullptr;
+    g->lchunks_cap = 0;
 }
 
 int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {

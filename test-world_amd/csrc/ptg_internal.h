@@ -170,6 +170,7 @@ struct ptg_graph {
 
     int symmetric = -1;  // -1 unknown, 0/1 once checked
     uint64_t device_bytes = 0;                          // charged against the budget
+    uint64_t create_bytes = 0;                          // device_bytes when creation finished
     std::unordered_map<void *, uint64_t> alloc_bytes;   // live allocations and their sizes
     bool weighted = false;
 };
@@ -191,6 +192,9 @@ void free_static_bins(ptg_graph *g, StaticBins *sb);
 // Kernel drivers (frontier.hip, pagerank.hip). `out` is a device pointer.
 int fill_u32(ptg_graph *g, uint32_t *p, uint64_t n, uint32_t v);
 int copy_bytes(void *dst, const void *src, uint64_t n, hipStream_t stream);
+// Copy on the graph's own stream, then wait for it. The legacy default stream
+// is never used: another thread may be capturing a graph replay meanwhile.
+int copy_sync(ptg_graph *g, void *dst, const void *src, uint64_t bytes, hipMemcpyKind kind);
 int run_bfs(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_cc(ptg_graph *g, uint32_t *out, ptg_stats *st);
@@ -204,6 +208,12 @@ int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t 
 int run_batch_query(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, uint32_t *out, ptg_stats *st);
 int batch_staging(ptg_graph *g, uint32_t **out);
 void destroy_batch(ptg_graph *g);
+// Serving support: bytes a query kind would still allocate, release of the
+// lazily allocated query state, and per-row checksums of n x V results.
+uint64_t batch_state_need(const ptg_graph *g, bool sssp);
+uint64_t pr_state_need(const ptg_graph *g);
+void release_pr_state(ptg_graph *g);
+int row_checksums(ptg_graph *g, hipStream_t stream, const uint32_t *rows, uint32_t n, unsigned long long *d_out);
 // dst[v] = src[perm[v]] for v < V (new-id results back to old ids).
 int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst);
 

@@ -37,6 +37,27 @@ bool is_device_pointer(const void *p) {
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
+int copy_sync(ptg_graph *g, void *dst, const void *src, uint64_t bytes, hipMemcpyKind kind) {
+    if (bytes == 0) return PTG_OK;
+    PTG_HIP(hipMemcpyAsync(dst, src, bytes, kind, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    return PTG_OK;
+}
+
+// A non-blocking stream of this thread for the device-buffer utilities.
+static int util_stream(hipStream_t *st) {
+    thread_local hipStream_t s = nullptr;
+    thread_local int dev = -1;
+    int cur = 0;
+    PTG_HIP(hipGetDevice(&cur));
+    if (!s || dev != cur) {
+        PTG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        dev = cur;
+    }
+    *st = s;
+    return PTG_OK;
+}
+
 int galloc(ptg_graph *g, void **p, uint64_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
@@ -101,8 +122,8 @@ int build_static_bins(ptg_graph *g, const uint32_t *h_off, uint32_t V, StaticBin
     auto put = [&](std::vector<uint32_t> &src, uint32_t *&dst) -> int {
         dst = p;
         if (!src.empty()) {
-            hipError_t e = hipMemcpy(p, src.data(), src.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-            if (e != hipSuccess) return hip_fail(e, "hipMemcpy(static bins)");
+            const int rc = copy_sync(g, p, src.data(), src.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+            if (rc) return rc;
         }
         p += src.size() + 1;
         return PTG_OK;
@@ -119,7 +140,7 @@ int build_static_bins(ptg_graph *g, const uint32_t *h_off, uint32_t V, StaticBin
     sb->h_ctr.l_pack = ((unsigned long long)l.size() << kLCountShift) | chunks;
     sb->h_ctr.found = (uint32_t)(s.size() + m.size() + l.size());
     sb->n_zero = (uint32_t)z.size();
-    PTG_HIP(hipMemcpy(sb->d_ctr, &sb->h_ctr, sizeof(BinCounters), hipMemcpyHostToDevice));
+    if (int rc = copy_sync(g, sb->d_ctr, &sb->h_ctr, sizeof(BinCounters), hipMemcpyHostToDevice)) return rc;
     return PTG_OK;
 }
 
@@ -274,13 +295,10 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     TRY(galloc(g, (void **)&g->out.off, sizeof(uint32_t) * ((uint64_t)V + 1)));
     TRY(galloc(g, (void **)&g->out.col, sizeof(uint32_t) * E));
     if (g->weighted) TRY(galloc(g, (void **)&g->d_w, sizeof(uint32_t) * E));
-    if (hipMemcpy(g->out.off, h_off.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice) != hipSuccess)
-        TRY(fail(PTG_ERR_HIP, "hipMemcpy(offsets)"));
+    TRY(copy_sync(g, g->out.off, h_off.data(), sizeof(uint32_t) * ((uint64_t)V + 1), hipMemcpyHostToDevice));
     if (up_col) {
-        if (E && hipMemcpy(g->out.col, up_col, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
-            TRY(fail(PTG_ERR_HIP, "hipMemcpy(cols)"));
-        if (up_w && E && hipMemcpy(g->d_w, up_w, sizeof(uint32_t) * E, hipMemcpyHostToDevice) != hipSuccess)
-            TRY(fail(PTG_ERR_HIP, "hipMemcpy(weights)"));
+        if (E) TRY(copy_sync(g, g->out.col, up_col, sizeof(uint32_t) * E, hipMemcpyHostToDevice));
+        if (up_w && E) TRY(copy_sync(g, g->d_w, up_w, sizeof(uint32_t) * E, hipMemcpyHostToDevice));
     } else if (E) {
         // AoS EdgeWithWeight -> SoA columns + weights, in host blocks.
         const uint64_t blk = 1ull << 22;
@@ -291,18 +309,16 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
                 tc[i] = wedges[e0 + i].toNode;
                 tw[i] = wedges[e0 + i].weight;
             }
-            if (hipMemcpy(g->out.col + e0, tc.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(g->d_w + e0, tw.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess)
-                TRY(fail(PTG_ERR_HIP, "hipMemcpy(edges)"));
+            TRY(copy_sync(g, g->out.col + e0, tc.data(), n * 4, hipMemcpyHostToDevice));
+            TRY(copy_sync(g, g->d_w + e0, tw.data(), n * 4, hipMemcpyHostToDevice));
         }
     }
     if (relabel) {
         TRY(galloc(g, (void **)&g->d_perm, sizeof(uint32_t) * V));
         TRY(galloc(g, (void **)&g->d_inv, sizeof(uint32_t) * V));
         TRY(galloc(g, (void **)&g->d_val2, sizeof(uint32_t) * V));
-        if (hipMemcpy(g->d_perm, rl.perm.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g->d_inv, rl.inv.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice) != hipSuccess)
-            TRY(fail(PTG_ERR_HIP, "hipMemcpy(permutation)"));
+        TRY(copy_sync(g, g->d_perm, rl.perm.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice));
+        TRY(copy_sync(g, g->d_inv, rl.inv.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice));
         g->h_perm.swap(rl.perm);
     }
     TRY(build_static_bins(g, h_off.data(), V, &g->out.bins));
@@ -335,6 +351,7 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
         TRY(check_symmetric(g, &sym));
     }
 #undef TRY
+    g->create_bytes = g->device_bytes;
     *out = g;
     return PTG_OK;
 }
@@ -450,12 +467,18 @@ int ptg_device_free(void *ptr) {
 }
 
 int ptg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
-    PTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    hipStream_t st;
+    if (int rc = util_stream(&st)) return rc;
+    PTG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    PTG_HIP(hipStreamSynchronize(st));
     return PTG_OK;
 }
 
 int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
-    PTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    hipStream_t st;
+    if (int rc = util_stream(&st)) return rc;
+    PTG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    PTG_HIP(hipStreamSynchronize(st));
     return PTG_OK;
 }
 
@@ -464,9 +487,10 @@ int ptg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
     if (!dst || !src) return fail(PTG_ERR_INVALID_ARG, "null pointer");
     if (!is_device_pointer(dst) || !is_device_pointer(src))
         return fail(PTG_ERR_INVALID_ARG, "ptg_memcpy_d2d needs two device pointers");
-    int rc = copy_bytes(dst, src, bytes, nullptr);
-    if (rc) return rc;
-    PTG_HIP(hipDeviceSynchronize());
+    hipStream_t st;
+    int rc = util_stream(&st);
+    if (rc || (rc = copy_bytes(dst, src, bytes, st))) return rc;
+    PTG_HIP(hipStreamSynchronize(st));
     return PTG_OK;
 }
 

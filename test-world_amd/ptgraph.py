@@ -30,6 +30,7 @@ FLAG_NO_VALIDATE = 0x1
 FLAG_TIMING = 0x2
 FLAG_NO_RELABEL = 0x4
 TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH = 0x1, 0x2, 0x4, 0x8
+Q_BFS, Q_SSSP, Q_CC, Q_PAGERANK = 0, 1, 2, 3
 
 BFS_UNREACHED = 0xFFFFFFFF  # ptGraph/main.cu:1068
 SSSP_SENTINEL_SHARE = 0xFFFFFFFE  # ptGraph/main.cu:584
@@ -92,6 +93,58 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("state_budget_bytes", ctypes.c_uint64),
+        ("max_batch", ctypes.c_uint32),
+        ("max_inflight", ctypes.c_uint32),
+        ("coalesce", ctypes.c_uint32),
+        ("queue_cap", ctypes.c_uint32),
+        ("min_batch", ctypes.c_uint32),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("graph", ctypes.c_uint32),
+        ("kind", ctypes.c_uint32),
+        ("source", ctypes.c_uint32),
+        ("iterations", ctypes.c_uint32),
+        ("damping", ctypes.c_float),
+        ("pad", ctypes.c_uint32),
+        ("out", ctypes.c_void_p),
+        ("tag", ctypes.c_uint64),
+    ]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [
+        ("tag", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("batch_size", ctypes.c_uint32),
+        ("checksum", ctypes.c_uint64),
+        ("submit_ns", ctypes.c_uint64),
+        ("start_ns", ctypes.c_uint64),
+        ("done_ns", ctypes.c_uint64),
+    ]
+
+
+class ServerSummary(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("p50_ns", ctypes.c_uint64),
+        ("p99_ns", ctypes.c_uint64),
+        ("max_ns", ctypes.c_uint64),
+        ("mean_batch", ctypes.c_double),
+        ("first_submit_ns", ctypes.c_uint64),
+        ("last_done_ns", ctypes.c_uint64),
+        ("state_bytes_peak", ctypes.c_uint64),
+        ("evictions", ctypes.c_uint64),
+    ]
+
+
 class _HostCsr(ctypes.Structure):
     _fields_ = [
         ("V", ctypes.c_uint32),
@@ -134,6 +187,17 @@ SIGNATURES = {
     "ptg_graph_set_flags": (ctypes.c_int, [_vp, ctypes.c_uint32]),
     "ptg_graph_set_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
     "ptg_graph_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
+    "ptg_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+    "ptg_server_add_graph": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_uint32)]),
+    "ptg_server_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Request)]),
+    "ptg_server_replay": (ctypes.c_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.c_uint64]),
+    "ptg_server_poll": (ctypes.c_int, [_vp, ctypes.POINTER(Completion), ctypes.c_int, ctypes.c_int]),
+    "ptg_server_drain": (ctypes.c_int, [_vp]),
+    "ptg_server_pause": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ptg_server_error": (ctypes.c_char_p, [_vp]),
+    "ptg_server_summary_get": (ctypes.c_int, [_vp, ctypes.POINTER(ServerSummary)]),
+    "ptg_server_destroy": (None, [_vp]),
     "ptg_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, ctypes.c_uint32,
                                     ctypes.POINTER(Config)]),
     "ptg_bfs": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
@@ -485,3 +549,90 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Request loop over resident graphs (ptg_server_*): continuous batching,
+    memory admission, latency summary."""
+
+    def __init__(self, state_budget_bytes=0, max_batch=64, max_inflight=0, coalesce=True, queue_cap=0, min_batch=0):
+        cfg = ServerConfig(state_budget_bytes, max_batch, max_inflight, 1 if coalesce else 0, queue_cap, min_batch)
+        h = _vp()
+        _check(lib().ptg_server_create(ctypes.byref(cfg), ctypes.byref(h)), "ptg_server_create")
+        self._h = h.value
+        self.graphs = []
+
+    def add_graph(self, graph):
+        gid = ctypes.c_uint32(0)
+        _check(lib().ptg_server_add_graph(self._h, graph._h, ctypes.byref(gid)), "ptg_server_add_graph")
+        self.graphs.append(graph)
+        return gid.value
+
+    @staticmethod
+    def request(graph, kind, source=0, iterations=10, damping=0.85, out=None, tag=0):
+        r = Request()
+        r.graph, r.kind, r.source, r.iterations, r.damping, r.tag = graph, kind, source, iterations, damping, tag
+        r.out = out
+        return r
+
+    def submit(self, req):
+        _check(lib().ptg_server_submit(self._h, ctypes.byref(req)), "ptg_server_submit")
+
+    def replay(self, reqs, arrival_ns):
+        arr = (Request * len(reqs))(*reqs)
+        t = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        _check(lib().ptg_server_replay(self._h, arr, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(reqs)),
+               "ptg_server_replay")
+
+    def drain(self):
+        _check(lib().ptg_server_drain(self._h), "ptg_server_drain")
+
+    def error(self):
+        return lib().ptg_server_error(self._h).decode()
+
+    def pause(self, paused=True):
+        _check(lib().ptg_server_pause(self._h, 1 if paused else 0), "ptg_server_pause")
+
+    def poll(self, max_n=4096, timeout_ms=0):
+        buf = (Completion * max_n)()
+        k = lib().ptg_server_poll(self._h, buf, max_n, timeout_ms)
+        if k < 0:
+            raise PtgError(k, "ptg_server_poll")
+        return [buf[i] for i in range(k)]
+
+    def completions(self):
+        out = []
+        while True:
+            c = self.poll()
+            if not c:
+                return out
+            out.extend(c)
+
+    def summary(self):
+        s = ServerSummary()
+        _check(lib().ptg_server_summary_get(self._h, ctypes.byref(s)), "ptg_server_summary_get")
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ptg_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def checksum(values):
+    """sum over v of values[v] * (2v + 1) mod 2^64 (the server's result checksum)."""
+    v = np.ascontiguousarray(values).view(np.uint32).astype(np.uint64)
+    k = 2 * np.arange(v.size, dtype=np.uint64) + np.uint64(1)
+    return int(np.sum(v * k, dtype=np.uint64))

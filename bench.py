@@ -22,6 +22,15 @@ its HIP-event-timed launch durations give `achieved`.
 cpu_baseline: the oracle's restatement of the reference's frontier loop
 (oracle/oracle.c, orc_ref_loop) on the host cores, BFS and SSSP from the
 same 8 sources (16 queries), as GTEPS over the same m_r.
+
+After the timed steps (none of this is in `value`):
+  batched  64 sources (seed 3) per launch on the same graph, batch knobs
+           picked by the autotuner, vs the same 64 single queries: queries/s.
+  serving  6 resident R-MAT graphs (scales 18..24), 3000 requests, mix 50%
+           BFS / 30% SSSP / 15% CC / 5% PageRank(10), Poisson arrivals at
+           400/s (seed 4): the plain per-query path (one at a time) vs the
+           server (continuous batching); p50/p99 at that load and queries/s
+           with every request queued at once; checksums must match.
 """
 import argparse
 import json
@@ -66,6 +75,104 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def batched_leg(g, srcs, V, pg, log):
+    """64 sources per launch vs the same 64 queries one at a time (outputs in
+    HBM, equal results): queries/s of each and their ratio."""
+    import numpy as np
+
+    n = len(srcs)
+    buf = pg.DeviceBuffer(4 * n * V)
+    one = pg.DeviceBuffer(4 * V)
+    res = {"sources": n, "graph": "the bench graph"}
+    try:
+        for kind in ("bfs", "sssp"):
+            fb = g.bfs_batch if kind == "bfs" else g.sssp_batch
+            f1 = g.bfs if kind == "bfs" else g.sssp
+            fb(srcs, out=buf)
+            f1(int(srcs[0]), out=one)
+            tb = []
+            for _ in range(3):
+                t = time.perf_counter()
+                fb(srcs, out=buf)
+                tb.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            for s in srcs:
+                f1(int(s), out=one)
+            t1 = time.perf_counter() - t
+            res[kind + "_batched_qps"] = round(n / min(tb), 1)
+            res[kind + "_plain_qps"] = round(n / t1, 1)
+            res[kind + "_ratio"] = round(t1 / min(tb), 2)
+        # spot check: one row of each batch against the single query
+        g.bfs_batch(srcs, out=buf)
+        k = n // 2
+        row = buf.to_host(np.uint32, n * V)[k * V:(k + 1) * V]
+        res["bfs_row_equal"] = bool((row == g.bfs(int(srcs[k]))[0]).all())
+        g.sssp_batch(srcs, out=buf)
+        row = buf.to_host(np.uint32, n * V)[k * V:(k + 1) * V]
+        res["sssp_row_equal"] = bool((row == g.sssp(int(srcs[k]))[0]).all())
+    finally:
+        buf.free()
+        one.free()
+    log("batched: %s" % res)
+    return res
+
+
+def serving_leg(args, pg, np, cores, log):
+    """Poisson stream (seed 4) over resident R-MAT graphs, mix 50% BFS / 30%
+    SSSP / 15% CC / 5% PageRank (10 iterations), uniform graphs and sources;
+    the plain per-query path (one query at a time, no batching) against the
+    server (continuous batching, coalescing, one worker per graph) at the same
+    offered load, then both saturated with every request at once."""
+    scales = [int(x) for x in args.serve_scales.split(",") if x]
+    graphs = []
+    t = time.time()
+    for sc in scales:
+        c = pg.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=1, threads=cores).add_weights(1, 64, seed=2, threads=cores)
+        graphs.append(pg.Graph.from_csr(c))
+        del c
+    log("serving: %d graphs (scales %s) ready in %.1fs" % (len(graphs), scales, time.time() - t))
+    rng = np.random.default_rng(4)
+    N = args.serve_requests
+    kinds = rng.choice([pg.Q_BFS, pg.Q_SSSP, pg.Q_CC, pg.Q_PAGERANK], N, p=[0.5, 0.3, 0.15, 0.05])
+    gids = rng.integers(len(graphs), size=N)
+    reqs = [pg.Server.request(int(gids[i]), int(kinds[i]), source=int(rng.integers(graphs[gids[i]].V)),
+                              iterations=10, damping=0.85, tag=i) for i in range(N)]
+    arrivals = np.cumsum(rng.exponential(1e9 / args.serve_qps, N)).astype(np.uint64)
+    out = {"graphs": ["rmat%d" % s for s in scales], "requests": N, "offered_qps": args.serve_qps,
+           "mix": "50% BFS, 30% SSSP, 15% CC, 5% PageRank(10)"}
+    sums = {}
+    for mode, kw in (("plain", dict(max_batch=1, max_inflight=1, coalesce=False)), ("served", {})):
+        with pg.Server(**kw) as srv:
+            for gr in graphs:
+                srv.add_graph(gr)
+            srv.replay(reqs[:300], np.zeros(300, np.uint64))  # warm: state, captures
+            srv.drain()
+            srv.completions()
+        res = {}
+        for phase, arr in (("offered", arrivals), ("saturated", np.zeros(N, np.uint64))):
+            with pg.Server(**kw) as srv:
+                for gr in graphs:
+                    srv.add_graph(gr)
+                t = time.perf_counter()
+                srv.replay(reqs, arr)
+                srv.drain()
+                el = time.perf_counter() - t
+                s = srv.summary()
+                comps = srv.completions()
+                if phase == "offered":
+                    sums[mode] = {c.tag: c.checksum for c in comps}
+                res[phase] = {"qps": round(s.completed / el, 1), "p50_ms": round(s.p50_ns / 1e6, 3),
+                              "p99_ms": round(s.p99_ns / 1e6, 3), "mean_batch": round(s.mean_batch, 2),
+                              "ok": int(sum(c.status == 0 for c in comps))}
+        out[mode] = res
+        log("serving %s: %s" % (mode, res))
+    out["saturated_qps_ratio"] = round(out["served"]["saturated"]["qps"] / out["plain"]["saturated"]["qps"], 2)
+    out["results_identical"] = sums["plain"] == sums["served"]
+    for gr in graphs:
+        gr.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +184,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not time main-kernel launches with HIP events (no roofline)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-source leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tuning (no autotuner run)")
+    ap.add_argument("--serve-scales", default="18,20,21,22,23,24")
+    ap.add_argument("--serve-qps", type=float, default=400.0)
+    ap.add_argument("--serve-requests", type=int, default=3000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB 3 times (hardware-counter calibration)")
     args = ap.parse_args()
@@ -115,6 +228,7 @@ def main():
     out = pg.DeviceBuffer(4 * V)
 
     kinds = ("bfs", "sssp", "cc", "pagerank")
+    srcs64 = np.array(rng.choice(np.nonzero(deg > 0)[0], 64, replace=False), dtype=np.uint32)
 
     def set_timing(kind, timed):
         on = not args.no_timing and (timed == "all" or kind == timed)
@@ -170,6 +284,17 @@ def main():
             _, st = fn(s)
             m += st.edges_traversed
         pcie[kind + "_gteps"] = round(m / (time.perf_counter() - t) / 1e9, 4)
+
+    # The single-query path keeps its default tuning (the profiles under
+    # profiles/ describe it); the autotuner picks the batched kernels' knobs.
+    tuned = None
+    if not args.no_tune and not args.no_batch:
+        t = time.time()
+        c = g.autotune(srcs64, what=pg.TUNE_BFS_BATCH | pg.TUNE_SSSP_BATCH)
+        tuned = {f: getattr(c, f) for f in ("batch_td_to_bu", "batch_grid", "batch_delta", "batch_chunk")}
+        log("autotuner %.1fs: %s" % (time.time() - t, tuned))
+    batched = None if args.no_batch else batched_leg(g, srcs64, V, pg, log)
+    serving = None if args.no_serve else serving_leg(args, pg, np, cores, log)
 
     def tot(k, idx):
         return sum(r[idx] for r in acc[k])
@@ -281,6 +406,9 @@ def main():
         "per_kernel_warmup": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in d.items()}
                               for k, d in per_kind_warm.items()},
         "host_output_gteps": pcie,
+        "tuning": tuned,
+        "batched": batched,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

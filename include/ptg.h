@@ -64,6 +64,12 @@ enum {
 #define PTG_FLAG_NO_VALIDATE 0x1u  /* skip the host-side CSR validation */
 #define PTG_FLAG_TIMING 0x2u       /* time every main-kernel launch with HIP events */
 #define PTG_FLAG_NO_RELABEL 0x4u   /* keep the caller's vertex order on the device */
+/* Keep the edges in pinned host memory, cut into chunks of at most
+ * chunk_bytes, and stream them into a device cache sized by
+ * device_budget_bytes (replaces the reference's partition-swap path,
+ * main.cu:1681-2059). Such a graph answers BFS and PageRank (symmetric
+ * graphs); other queries return PTG_ERR_UNSUPPORTED. */
+#define PTG_FLAG_STREAM 0x8u
 
 typedef struct ptg_config {
     uint64_t device_budget_bytes; /* 0 = whole device; cap on graph + state */
@@ -95,7 +101,9 @@ typedef struct ptg_stats {
     uint64_t main_kernel_ns;  /* PTG_FLAG_TIMING: summed HIP-event time of the query's
                                  main kernel (frontier expansion / PageRank sweep) */
     uint32_t main_launches;   /* PTG_FLAG_TIMING: launches of that kernel */
-    uint32_t reserved;
+    uint32_t chunk_loads;     /* PTG_FLAG_STREAM: chunks copied from the host during the query */
+    uint32_t chunk_hits;      /* PTG_FLAG_STREAM: chunk uses served from the device cache */
+    uint32_t pad;
 } ptg_stats;
 
 typedef struct ptg_graph ptg_graph;
@@ -127,6 +135,12 @@ PTG_API int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
  * byte counters read exactly 2 x bytes of algorithmic traffic, which is what
  * calibrates hardware counters. */
 PTG_API int ptg_memcpy_d2d(void *dst, const void *src, uint64_t bytes);
+
+/* PTG_FLAG_STREAM graphs: number of chunks, device cache slots and bytes per
+ * slot (PTG_ERR_UNSUPPORTED for a resident graph). */
+PTG_API int ptg_graph_stream_info(const ptg_graph *g, uint32_t *chunks, uint32_t *slots, uint64_t *slot_bytes);
+/* Pinned host -> device copy rate in GB/s (`reps` copies of `bytes`). */
+PTG_API int ptg_host_link_probe(uint64_t bytes, int reps, double *gbs);
 
 /* ---- graph ---------------------------------------------------------------- */
 /* Checks the reference-layout CSR on the host: offsets non-decreasing,

@@ -110,7 +110,8 @@ struct Csr {
 }  // namespace ptg
 
 namespace ptg {
-struct BatchState;  // batch.hip
+struct BatchState;   // batch.hip
+struct StreamState;  // stream.hip
 }
 
 // The graph handle.
@@ -167,6 +168,8 @@ struct ptg_graph {
 
     // Batched multi-source state (batch.hip), allocated on the first batch.
     ptg::BatchState *batch = nullptr;
+    // PTG_FLAG_STREAM: host-resident edges and the device chunk cache (stream.hip).
+    ptg::StreamState *streamed = nullptr;
 
     int symmetric = -1;  // -1 unknown, 0/1 once checked
     uint64_t device_bytes = 0;                          // charged against the budget
@@ -214,6 +217,13 @@ uint64_t batch_state_need(const ptg_graph *g, bool sssp);
 uint64_t pr_state_need(const ptg_graph *g);
 void release_pr_state(ptg_graph *g);
 int row_checksums(ptg_graph *g, hipStream_t stream, const uint32_t *rows, uint32_t n, unsigned long long *d_out);
+// Streamed graphs (stream.hip).
+int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges);
+void destroy_streamed(ptg_graph *g);
+int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st);
+int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st);
+int host_link_probe(uint64_t bytes, int reps, double *gbs);
+int stream_info(const ptg_graph *g, uint32_t *chunks, uint32_t *slots, uint64_t *slot_bytes);
 // dst[v] = src[perm[v]] for v < V (new-id results back to old ids).
 int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst);
 

@@ -213,6 +213,7 @@ static void destroy(ptg_graph *g) {
     gfree(g, g->d_acc);
     for (auto *c : g->d_contrib) gfree(g, c);
     destroy_batch(g);
+    destroy_streamed(g);
     gfree(g, g->d_part);
     gfree(g, g->d_lpart);
     if (g->h_ring) (void)hipHostFree(g->h_ring);
@@ -243,8 +244,9 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
         return fail(PTG_ERR_INVALID_ARG, "graph_create: bad sizes or null arrays");
     }
     const bool weighted = (weights != nullptr) || (wedges != nullptr);
+    const bool streamed = (c.flags & PTG_FLAG_STREAM) != 0;
     const uint64_t need = ptg_graph_estimate_bytes(V, E, weighted, &c);
-    if (c.device_budget_bytes && need > c.device_budget_bytes)
+    if (!streamed && c.device_budget_bytes && need > c.device_budget_bytes)
         return fail(PTG_ERR_BUDGET, "graph needs " + std::to_string(need) + " device bytes, budget is " +
                                         std::to_string(c.device_budget_bytes));
     int ndev = 0;
@@ -275,6 +277,13 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
         TRY(fail(PTG_ERR_HIP, "hipStreamCreateWithFlags"));
     if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
         TRY(fail(PTG_ERR_HIP, "hipEventCreate"));
+    if (streamed) {
+        g->weighted = false;  // a streamed graph keeps columns only (BFS, PageRank)
+        TRY(create_streamed(g, ptr, cols, wedges));
+        g->create_bytes = g->device_bytes;
+        *out = g;
+        return PTG_OK;
+    }
 
     // CSR in HBM: V+1 offsets, relabeled by degree unless asked not to.
     std::vector<uint32_t> h_off;
@@ -482,6 +491,17 @@ int ptg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
     return PTG_OK;
 }
 
+int ptg_graph_stream_info(const ptg_graph *g, uint32_t *chunks, uint32_t *slots, uint64_t *slot_bytes) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (!g->streamed) return fail(PTG_ERR_UNSUPPORTED, "not a streamed graph");
+    return stream_info(g, chunks, slots, slot_bytes);
+}
+
+int ptg_host_link_probe(uint64_t bytes, int reps, double *gbs) {
+    if (!gbs || bytes == 0 || reps < 1) return fail(PTG_ERR_INVALID_ARG, "host_link_probe: bad arguments");
+    return host_link_probe(bytes, reps, gbs);
+}
+
 int ptg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
     if (bytes == 0) return PTG_OK;
     if (!dst || !src) return fail(PTG_ERR_INVALID_ARG, "null pointer");
@@ -571,12 +591,14 @@ int ptg_bfs(ptg_graph *g, uint32_t src, uint32_t *levels_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "bfs: source >= V");
     const uint32_t s = map_source(g, src);
+    if (g->streamed) return with_output(g, levels_out, [&](uint32_t *o) { return run_bfs_streamed(g, s, o, st); });
     return with_output(g, levels_out, [&](uint32_t *o) { return run_bfs(g, s, o, st); });
 }
 
 int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "sssp: source >= V");
+    if (g->streamed) return fail(PTG_ERR_UNSUPPORTED, "sssp: not available on a streamed graph");
     if (!g->weighted) return fail(PTG_ERR_UNSUPPORTED, "sssp: graph has no weights");
     const uint32_t s = map_source(g, src);
     return with_output(g, dist_out, [&](uint32_t *o) { return run_sssp(g, s, o, st); });
@@ -584,6 +606,7 @@ int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *st) {
 
 int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (g->streamed) return fail(PTG_ERR_UNSUPPORTED, "cc: not available on a streamed graph");
     return with_output(g, labels_out, [&](uint32_t *o) { return run_cc(g, o, st); });
 }
 
@@ -591,6 +614,8 @@ int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out, 
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (iterations < 0 || !(damping >= 0.0f && damping <= 1.0f))
         return fail(PTG_ERR_INVALID_ARG, "pagerank: need iterations >= 0 and 0 <= damping <= 1");
+    if (g->streamed)
+        return with_output(g, ranks_out, [&](float *o) { return run_pagerank_streamed(g, damping, iterations, o, st); });
     return with_output(g, ranks_out, [&](float *o) { return run_pagerank(g, damping, iterations, o, st); });
 }
 
@@ -601,6 +626,7 @@ static int batch_query(ptg_graph *g, bool sssp, const uint32_t *srcs, int n, uin
     if (n < 1 || n > 64) return fail(PTG_ERR_INVALID_ARG, "a batch holds 1 to 64 sources");
     for (int i = 0; i < n; i++)
         if (srcs[i] >= g->V) return fail(PTG_ERR_INVALID_ARG, "source out of range");
+    if (g->streamed) return fail(PTG_ERR_UNSUPPORTED, "batched queries are not available on a streamed graph");
     if (sssp && !g->weighted) return fail(PTG_ERR_UNSUPPORTED, "SSSP needs a weighted graph");
     ptg_stats local;
     if (!st) st = &local;

@@ -124,6 +124,7 @@ std::vector<Pending> form_launch(ptg_server *s, Slot *sl) {
         return true;
     };
     uint32_t limit = batchable ? std::max<uint32_t>(1, s->cfg.max_batch) : (s->cfg.coalesce ? ~0u : 1u);
+    if (sl->g->streamed) limit = batchable ? 1u : limit;  // a streamed graph has no batched path
     if (batchable && limit > 1) {
         // a batched launch costs about the same for any number of sources:
         // below min_batch pending requests, serve the oldest one alone

@@ -1,0 +1,847 @@
+// Explicit CSR-chunk residency: graphs whose edges do not fit the device
+// budget (PTG_FLAG_STREAM).
+//
+// Replaces the reference's partition-swap path (bfsCaculateInAsyncSwap,
+// ptGraph/main.cu:1681-2059: a static prefix of the edge list in device
+// memory, the rest behind unified memory with per-iteration cudaMemAdvise
+// toggling and a host-thread gather of the overloaded adjacency lists,
+// main.cu:1126-1156). Here the columns stay in pinned host memory cut into
+// vertex-aligned chunks of at most chunk_bytes; the device holds the vertex
+// state and a cache of chunk slots sized by the budget. A search step lists
+// the chunks its frontier touches, expands the resident ones first and
+// streams the others with hipMemcpyAsync on a copy stream, each kernel
+// waiting on its chunk's copy event and each copy on the event that frees
+// its slot, so copies overlap with the expansion of earlier chunks.
+//
+// Vertices with more than kHubEdges edges are cut into pieces of kHubEdges
+// (one wave each) so that no wave walks a hub's whole list; PageRank sums a
+// hub's pieces in a fixed order afterwards. BFS is top-down; PageRank pulls
+// over the out-lists, which equal the in-lists on a symmetric graph.
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "device_util.h"
+#include "relabel.h"
+
+namespace ptg {
+
+constexpr uint32_t kHubEdges = 4096;
+constexpr uint32_t kStreamGrid = 1024;  // workgroups of the chunk kernels (fixed: partial slots)
+
+struct ChunkDesc {
+    uint64_t eb, ee;   // edge range
+    uint32_t vb, vh;   // first vertex, first non-hub vertex
+    uint32_t ve;       // one past the last vertex
+    uint32_t pb, pe;   // hub pieces of the chunk
+    uint32_t rb, re;   // hub runs (one per hub vertex) of the chunk
+};
+
+struct StreamState {
+    uint32_t *h_col = nullptr;  // pinned host columns
+    std::vector<ChunkDesc> chunks;
+    uint64_t max_chunk_edges = 0;
+    // pieces (global edge sub-ranges of hub vertices) and runs (one per hub)
+    uint32_t *d_pv = nullptr, *d_pe0 = nullptr, *d_pe1 = nullptr;
+    uint32_t *d_rv = nullptr, *d_rpb = nullptr, *d_rpe = nullptr;
+    uint32_t npieces = 0, nruns = 0;
+    uint32_t split_rb = 0;      // runs [split_rb, nruns) belong to vertices larger than a chunk
+    uint64_t zeros = 0;         // vertices without out-edges
+    uint32_t *d_cvb = nullptr, *d_cve = nullptr, *d_need = nullptr, *h_need = nullptr;
+    std::vector<uint32_t *> slot;
+    std::vector<int> slot_chunk;
+    std::vector<uint64_t> slot_used;
+    std::vector<int> chunk_slot;
+    std::vector<hipEvent_t> slot_free, slot_ready;
+    std::vector<hipEvent_t> tcopy;  // start/stop pairs of the copies of one query
+    uint32_t tcopy_used = 0;
+    hipStream_t copy = nullptr;
+    uint32_t *d_bits[2] = {};
+    unsigned long long *d_cnt = nullptr, *h_cnt = nullptr;  // 2 x kShards
+    float *d_contrib[2] = {};
+    float *d_ppart = nullptr;  // per piece (hi, lo)
+    float *d_dpart = nullptr;  // 2 iterations x chunks x kStreamGrid x (hi, lo)
+    float *d_base = nullptr;   // this iteration's base value + D0 (hi, lo)
+    uint64_t clock = 0;
+    uint64_t loads = 0, hits = 0, bytes = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// need[c] = 1 for every chunk holding a frontier vertex; cnt[0] += frontier size.
+__global__ __launch_bounds__(kBlock) void k_stream_need(uint32_t nwords, const uint32_t *__restrict__ bits,
+                                                        uint32_t nch, const uint32_t *__restrict__ cvb,
+                                                        const uint32_t *__restrict__ cve, uint32_t *need,
+                                                        unsigned long long *cnt) {
+    uint64_t f = 0;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+        uint32_t m = bits[w];
+        f += __popc(m);
+        while (m) {
+            const uint32_t v = w * 32 + __builtin_ctz(m);
+            m &= m - 1;
+            // chunks holding v (vb <= v < ve) form a run starting at the first
+            // chunk with ve > v
+            uint32_t lo = 0, hi = nch;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (cve[mid] <= v) lo = mid + 1; else hi = mid;
+            }
+            for (uint32_t c = lo; c < nch && cvb[c] <= v; c++) need[c] = 1u;
+        }
+    }
+    f = wave_sum_u64(f);
+    if (lane_id() == 0 && f) atomicAdd(&cnt[blockIdx.x % kShards], (unsigned long long)f);
+}
+
+struct ChunkArgs {
+    uint64_t eb;
+    uint32_t vh, ve;
+    uint32_t pb, pe;
+    const uint32_t *slot;  // columns eb.. of the chunk
+};
+
+__device__ __forceinline__ void bfs_visit(uint32_t w, uint32_t *level, uint32_t *nxt, uint32_t nl, uint64_t &found) {
+    if (level[w] == 0xFFFFFFFFu) {
+        level[w] = nl;
+        const uint32_t bit = 1u << (w & 31);
+        if (!(atomicOr(&nxt[w >> 5], bit) & bit)) found++;
+    }
+}
+
+// Top-down expansion of the frontier vertices of one resident chunk.
+__global__ __launch_bounds__(kBlock) void k_stream_bfs(ChunkArgs c, const uint32_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ pv,
+                                                       const uint32_t *__restrict__ pe0,
+                                                       const uint32_t *__restrict__ pe1, uint32_t *level,
+                                                       const uint32_t *__restrict__ cur, uint32_t *nxt, uint32_t nl,
+                                                       unsigned long long *cnt) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    uint64_t found = 0;
+    // hub pieces: one wave each
+    for (uint32_t i = c.pb + gw; i < c.pe; i += nw) {
+        const uint32_t v = pv[i];
+        if (!((cur[v >> 5] >> (v & 31)) & 1u)) continue;
+        const uint64_t e1 = pe1[i];
+        for (uint64_t e = pe0[i] + lane; e < e1; e += 64) bfs_visit(c.slot[e - c.eb], level, nxt, nl, found);
+    }
+    // other vertices: 64 per wave step
+    for (uint64_t base = (uint64_t)c.vh + (uint64_t)gw * 64; base < c.ve; base += (uint64_t)nw * 64) {
+        const uint64_t v = base + lane;
+        const bool act = v < c.ve && ((cur[v >> 5] >> (v & 31)) & 1u);
+        uint64_t b = 0, d = 0;
+        if (act) {
+            b = off[v];
+            d = off[v + 1] - b;
+        }
+        uint64_t mm = ballot(act && d > 32);
+        while (mm) {
+            const uint32_t q = lowest_lane(mm);
+            mm &= mm - 1;
+            const uint64_t bq = read_lane64(b, q), dq = read_lane64(d, q);
+            for (uint64_t k = lane; k < dq; k += 64) bfs_visit(c.slot[bq + k - c.eb], level, nxt, nl, found);
+        }
+        if (act && d <= 32)
+            for (uint64_t k = 0; k < d; k++) bfs_visit(c.slot[b + k - c.eb], level, nxt, nl, found);
+    }
+    found = wave_sum_u64(found);
+    if (lane == 0 && found) atomicAdd(&cnt[blockIdx.x % kShards], (unsigned long long)found);
+}
+
+// Compensated FP32 sums (TwoSum) in a fixed order.
+struct SCF {
+    float hi, lo;
+};
+
+__device__ __forceinline__ void scf_add(SCF &a, float x) {
+    const float s = a.hi + x;
+    const float bb = s - a.hi;
+    a.lo += (a.hi - (s - bb)) + (x - bb);
+    a.hi = s;
+}
+
+__device__ __forceinline__ SCF scf_merge(SCF a, SCF b) {
+    const float s = a.hi + b.hi;
+    const float bb = s - a.hi;
+    return SCF{s, (a.lo + b.lo) + ((a.hi - (s - bb)) + (b.hi - bb))};
+}
+
+__device__ __forceinline__ SCF wave_scf_sum(SCF a) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        SCF b;
+        b.hi = __shfl_xor(a.hi, d, 64);
+        b.lo = __shfl_xor(a.lo, d, 64);
+        a = scf_merge(a, b);
+    }
+    return a;
+}
+
+struct PrStep {
+    const uint32_t *off;
+    const float *cin;
+    float *cout;
+    float *rank;       // last iteration only
+    const float *base; // [0] = base of this iteration
+    float damping;
+};
+
+__device__ __forceinline__ void pr_finish(const PrStep &p, uint32_t v, float sum, float base, SCF &dang) {
+    const float r = base + p.damping * sum;
+    const uint32_t od = p.off[v + 1] - p.off[v];
+    p.cout[v] = od ? r / (float)od : 0.0f;
+    if (p.rank) p.rank[v] = r;
+    if (!od) scf_add(dang, r);
+}
+
+// PageRank over one resident chunk: hub pieces leave partial sums, every
+// other vertex of the chunk is finished here; the workgroup's dangling mass
+// goes to its fixed slot.
+__global__ __launch_bounds__(kBlock) void k_stream_pr(ChunkArgs c, PrStep p, const uint32_t *__restrict__ pe0,
+                                                      const uint32_t *__restrict__ pe1, float *ppart,
+                                                      float *dpart) {
+    __shared__ SCF lds[kWavesPerBlock];
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const float base = p.base[0];
+    for (uint32_t i = c.pb + gw; i < c.pe; i += nw) {
+        SCF a{0.0f, 0.0f};
+        const uint64_t e1 = pe1[i];
+        for (uint64_t e = pe0[i] + lane; e < e1; e += 64) scf_add(a, p.cin[c.slot[e - c.eb]]);
+        a = wave_scf_sum(a);
+        if (lane == 0) {
+            ppart[2ull * i] = a.hi;
+            ppart[2ull * i + 1] = a.lo;
+        }
+    }
+    SCF dang{0.0f, 0.0f};
+    for (uint64_t base_v = (uint64_t)c.vh + (uint64_t)gw * 64; base_v < c.ve; base_v += (uint64_t)nw * 64) {
+        const uint64_t v = base_v + lane;
+        const bool inb = v < c.ve;
+        uint64_t b = 0, d = 0;
+        if (inb) {
+            b = p.off[v];
+            d = p.off[v + 1] - b;
+        }
+        SCF mine{0.0f, 0.0f};
+        uint64_t mm = ballot(inb && d > 32);
+        while (mm) {
+            const uint32_t q = lowest_lane(mm);
+            mm &= mm - 1;
+            const uint64_t bq = read_lane64(b, q), dq = read_lane64(d, q);
+            SCF a{0.0f, 0.0f};
+            for (uint64_t k = lane; k < dq; k += 64) scf_add(a, p.cin[c.slot[bq + k - c.eb]]);
+            a = wave_scf_sum(a);
+            if (lane == q) mine = a;
+        }
+        if (inb && d <= 32)
+            for (uint64_t k = 0; k < d; k++) scf_add(mine, p.cin[c.slot[b + k - c.eb]]);
+        if (inb) pr_finish(p, (uint32_t)v, mine.hi + mine.lo, base, dang);
+    }
+    dang = wave_scf_sum(dang);
+    if (lane == 0) lds[threadIdx.x >> 6] = dang;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SCF t = lds[0];
+        for (int w = 1; w < kWavesPerBlock; w++) t = scf_merge(t, lds[w]);
+        dpart[2ull * blockIdx.x] = t.hi;
+        dpart[2ull * blockIdx.x + 1] = t.lo;
+    }
+}
+
+// Finishes the hub vertices of one chunk: one wave per run of pieces.
+__global__ __launch_bounds__(kBlock) void k_stream_pr_hubs(uint32_t rb, uint32_t re, const uint32_t *__restrict__ rv,
+                                                           const uint32_t *__restrict__ rpb,
+                                                           const uint32_t *__restrict__ rpe, const float *ppart,
+                                                           PrStep p) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const float base = p.base[0];
+    for (uint32_t r = rb + gw; r < re; r += nw) {
+        SCF a{0.0f, 0.0f};
+        for (uint32_t i = rpb[r] + lane; i < rpe[r]; i += 64) a = scf_merge(a, SCF{ppart[2ull * i], ppart[2ull * i + 1]});
+        a = wave_scf_sum(a);
+        if (lane == 0) {
+            SCF unused{0.0f, 0.0f};
+            pr_finish(p, rv[r], a.hi + a.lo, base, unused);  // hubs have out-edges: never dangling
+        }
+    }
+}
+
+// base = (1-d)/V + d*D/V with D summed over every chunk's workgroup slots in
+// index order (or D0 for the first iteration). One workgroup.
+__global__ __launch_bounds__(kBlock) void k_stream_pr_base(const float *dpart, uint32_t nslots, const float *d0,
+                                                           float damping, float inv_v, float *base) {
+    __shared__ SCF lds[kWavesPerBlock];
+    SCF t{0.0f, 0.0f};
+    if (!d0)
+        for (uint32_t j = threadIdx.x; j < nslots; j += kBlock) t = scf_merge(t, SCF{dpart[2ull * j], dpart[2ull * j + 1]});
+    t = wave_scf_sum(t);
+    if (lane_id() == 0) lds[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SCF s = lds[0];
+        for (int w = 1; w < kWavesPerBlock; w++) s = scf_merge(s, lds[w]);
+        const float dang = d0 ? d0[0] + d0[1] : s.hi + s.lo;
+        base[0] = (1.0f - damping) * inv_v + damping * (dang * inv_v);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_stream_pr_init(uint32_t V, const uint32_t *__restrict__ off, float r0,
+                                                           float *__restrict__ contrib) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const uint32_t od = off[v + 1] - off[v];
+        contrib[v] = od ? r0 / (float)od : 0.0f;
+    }
+}
+
+__global__ void k_stream_seed(uint32_t *level, uint32_t *bits, uint32_t src) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        level[src] = 1u;
+        bits[src >> 5] = 1u << (src & 31);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// Cuts the CSR into vertex-aligned chunks of at most max_edges edges (a
+// vertex with more edges than that gets chunks of its own) and the hub
+// vertices (more than kHubEdges edges) into pieces.
+static void plan_chunks(uint32_t V, const uint32_t *off, uint64_t max_edges, std::vector<ChunkDesc> *chunks,
+                        std::vector<uint32_t> *pv, std::vector<uint32_t> *pe0, std::vector<uint32_t> *pe1,
+                        std::vector<uint32_t> *rv, std::vector<uint32_t> *rpb, std::vector<uint32_t> *rpe,
+                        std::vector<uint32_t> *sv, std::vector<uint32_t> *spb, std::vector<uint32_t> *spe) {
+    auto deg = [&](uint32_t v) -> uint64_t { return (uint64_t)off[v + 1] - off[v]; };
+    auto add_pieces = [&](uint32_t v, uint64_t e0, uint64_t e1) {
+        for (uint64_t x = e0; x < e1; x += kHubEdges) {
+            pv->push_back(v);
+            pe0->push_back((uint32_t)x);
+            pe1->push_back((uint32_t)std::min<uint64_t>(x + kHubEdges, e1));
+        }
+    };
+    auto add_hub = [&](uint32_t v, uint64_t e0, uint64_t e1) {
+        rv->push_back(v);
+        rpb->push_back((uint32_t)pv->size());
+        add_pieces(v, e0, e1);
+        rpe->push_back((uint32_t)pv->size());
+    };
+    uint32_t v = 0;
+    while (v < V) {
+        ChunkDesc c{};
+        if (deg(v) > max_edges) {
+            // a vertex larger than a chunk: chunks of its own; its pieces
+            // span them all and are summed once every chunk has run
+            sv->push_back(v);
+            spb->push_back((uint32_t)pv->size());
+            for (uint64_t x = off[v]; x < off[v + 1]; x += max_edges) {
+                ChunkDesc s{};
+                s.eb = x;
+                s.ee = std::min<uint64_t>(x + max_edges, off[v + 1]);
+                s.vb = v;
+                s.vh = v + 1;
+                s.ve = v + 1;
+                s.pb = (uint32_t)pv->size();
+                add_pieces(v, s.eb, s.ee);
+                s.pe = (uint32_t)pv->size();
+                s.rb = s.re = (uint32_t)rv->size();
+                chunks->push_back(s);
+            }
+            spe->push_back((uint32_t)pv->size());
+            v++;
+            continue;
+        }
+        c.eb = off[v];
+        c.vb = v;
+        c.pb = (uint32_t)pv->size();
+        c.rb = (uint32_t)rv->size();
+        while (v < V && deg(v) <= max_edges && (uint64_t)off[v + 1] - c.eb <= max_edges) v++;
+        c.ve = v;
+        c.ee = off[v];
+        // hub vertices of the chunk: with degree ordering they lead the chunk;
+        // otherwise the chunk keeps hubs as ordinary vertices up to the first non-hub
+        uint32_t h = c.vb;
+        while (h < c.ve && deg(h) > kHubEdges) {
+            add_hub(h, off[h], off[h + 1]);
+            h++;
+        }
+        c.vh = h;
+        c.pe = (uint32_t)pv->size();
+        c.re = (uint32_t)rv->size();
+        chunks->push_back(c);
+    }
+}
+
+// Fingerprint test of symmetry over the host CSR: equal in- and out-degrees
+// and equal sums of an asymmetric 64-bit hash over (u, v) and (v, u).
+static bool host_symmetric(uint32_t V, const uint32_t *off, const uint32_t *col) {
+    const int T = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    auto h = [](uint64_t x) {
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    };
+    std::vector<unsigned long long> fwd(T, 0), bwd(T, 0);
+    std::vector<uint32_t> in((size_t)V, 0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            const uint32_t b = (uint32_t)((uint64_t)V * t / T), e = (uint32_t)((uint64_t)V * (t + 1) / T);
+            unsigned long long f = 0, r = 0;
+            for (uint32_t u = b; u < e; u++)
+                for (uint32_t k = off[u]; k < off[u + 1]; k++) {
+                    const uint32_t v = col[k];
+                    f += h(((uint64_t)u << 32) | v);
+                    r += h(((uint64_t)v << 32) | u);
+                    __atomic_fetch_add(&in[v], 1u, __ATOMIC_RELAXED);
+                }
+            fwd[t] = f;
+            bwd[t] = r;
+        });
+    for (auto &x : th) x.join();
+    unsigned long long F = 0, R = 0;
+    for (int t = 0; t < T; t++) {
+        F += fwd[t];
+        R += bwd[t];
+    }
+    if (F != R) return false;
+    for (uint32_t u = 0; u < V; u++)
+        if (in[u] != off[u + 1] - off[u]) return false;
+    return true;
+}
+
+int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges) {
+    const uint32_t V = g->V;
+    const uint64_t E = g->E;
+    const ptg_config &c = g->cfg;
+    int rc;
+    StreamState *s = new StreamState();
+    g->streamed = s;
+    // host CSR (relabeled by degree unless asked not to), columns pinned
+    std::vector<uint32_t> off;
+    Relabeled rl;
+    const uint32_t *col = cols;
+    std::vector<uint32_t> tmp;
+    if (!cols && wedges) {
+        tmp.resize(E);
+        for (uint64_t e = 0; e < E; e++) tmp[e] = wedges[e].toNode;
+        col = tmp.data();
+    }
+    if (!(c.flags & PTG_FLAG_NO_RELABEL)) {
+        relabel_by_degree(V, E, ptr, col, nullptr, nullptr, &rl);
+        off.swap(rl.off);
+        col = rl.col.data();
+        g->relabeled = true;
+    } else {
+        off.resize((size_t)V + 1);
+        memcpy(off.data(), ptr, 4ull * V);
+        off[V] = (uint32_t)E;
+    }
+    if (hipHostMalloc((void **)&s->h_col, 4 * std::max<uint64_t>(E, 1), hipHostMallocDefault) != hipSuccess)
+        return fail(PTG_ERR_OUT_OF_MEMORY, "pinned host columns");
+    if (E) memcpy(s->h_col, col, 4 * E);
+    g->symmetric = host_symmetric(V, off.data(), s->h_col) ? 1 : 0;
+    tmp.clear();
+    tmp.shrink_to_fit();
+    rl.col.clear();
+    rl.col.shrink_to_fit();
+    // chunk plan
+    s->max_chunk_edges = std::max<uint64_t>(64, (c.chunk_bytes ? c.chunk_bytes : (256ull << 20)) / 4);
+    std::vector<uint32_t> pv, pe0, pe1, rv, rpb, rpe, sv, spb, spe;
+    plan_chunks(V, off.data(), s->max_chunk_edges, &s->chunks, &pv, &pe0, &pe1, &rv, &rpb, &rpe, &sv, &spb, &spe);
+    // split-vertex runs go after the per-chunk runs
+    s->split_rb = (uint32_t)rv.size();
+    rv.insert(rv.end(), sv.begin(), sv.end());
+    rpb.insert(rpb.end(), spb.begin(), spb.end());
+    rpe.insert(rpe.end(), spe.begin(), spe.end());
+    s->zeros = 0;
+    for (uint32_t v = 0; v < V; v++) s->zeros += off[v + 1] == off[v];
+    const uint32_t C = (uint32_t)s->chunks.size();
+    s->npieces = (uint32_t)pv.size();
+    s->nruns = (uint32_t)rv.size();
+    uint64_t largest = 1;
+    for (auto &ch : s->chunks) largest = std::max<uint64_t>(largest, ch.ee - ch.eb);
+    // device vertex state
+    auto upload = [&](uint32_t **d, const std::vector<uint32_t> &h) -> int {
+        int r = galloc(g, (void **)d, 4 * std::max<size_t>(h.size(), 1));
+        if (!r && !h.empty()) r = copy_sync(g, *d, h.data(), 4 * h.size(), hipMemcpyHostToDevice);
+        return r;
+    };
+    std::vector<uint32_t> cvb(C), cve(C);
+    for (uint32_t i = 0; i < C; i++) {
+        cvb[i] = s->chunks[i].vb;
+        cve[i] = s->chunks[i].ve;
+    }
+    if ((rc = upload(&g->out.off, off)) || (rc = upload(&s->d_pv, pv)) || (rc = upload(&s->d_pe0, pe0)) ||
+        (rc = upload(&s->d_pe1, pe1)) || (rc = upload(&s->d_rv, rv)) || (rc = upload(&s->d_rpb, rpb)) ||
+        (rc = upload(&s->d_rpe, rpe)) || (rc = upload(&s->d_cvb, cvb)) || (rc = upload(&s->d_cve, cve)))
+        return rc;
+    const uint32_t nwords = (V + 31) / 32;
+    if ((rc = galloc(g, (void **)&s->d_need, 4 * std::max(C, 1u))) ||
+        (rc = galloc(g, (void **)&s->d_bits[0], 4ull * nwords)) ||
+        (rc = galloc(g, (void **)&s->d_bits[1], 4ull * nwords)) ||
+        (rc = galloc(g, (void **)&s->d_cnt, 16 * kShards)) || (rc = galloc(g, (void **)&g->d_val, 4ull * V)) ||
+        (rc = galloc(g, (void **)&g->d_acc, 16 * kShards)) ||
+        (rc = galloc(g, (void **)&s->d_contrib[0], 4ull * V)) ||
+        (rc = galloc(g, (void **)&s->d_contrib[1], 4ull * V)) ||
+        (rc = galloc(g, (void **)&s->d_ppart, 8ull * std::max(s->npieces, 1u))) ||
+        (rc = galloc(g, (void **)&s->d_dpart, 8ull * 2 * std::max(C, 1u) * kStreamGrid)) ||
+        (rc = galloc(g, (void **)&s->d_base, 16)))
+        return rc;
+    if (g->relabeled) {
+        if ((rc = galloc(g, (void **)&g->d_perm, 4ull * V)) || (rc = galloc(g, (void **)&g->d_inv, 4ull * V)) ||
+            (rc = galloc(g, (void **)&g->d_val2, 4ull * V)) ||
+            (rc = copy_sync(g, g->d_perm, rl.perm.data(), 4ull * V, hipMemcpyHostToDevice)) ||
+            (rc = copy_sync(g, g->d_inv, rl.inv.data(), 4ull * V, hipMemcpyHostToDevice)))
+            return rc;
+        g->h_perm.swap(rl.perm);
+    }
+    if (hipHostMalloc((void **)&s->h_need, 4 * std::max(C, 1u), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_cnt, 16 * kShards, hipHostMallocDefault) != hipSuccess)
+        return fail(PTG_ERR_OUT_OF_MEMORY, "pinned counters");
+    // chunk slots: what the budget leaves, at least two, at most one per chunk
+    const uint64_t slot_bytes = al256(4 * largest);
+    uint64_t nslots = C;
+    if (c.device_budget_bytes) {
+        const uint64_t left = c.device_budget_bytes > g->device_bytes ? c.device_budget_bytes - g->device_bytes : 0;
+        nslots = std::min<uint64_t>(C, left / slot_bytes);
+    }
+    if (nslots < std::min<uint64_t>(2, C))
+        return fail(PTG_ERR_BUDGET, "device budget leaves room for fewer than two chunk slots");
+    s->slot.assign(nslots, nullptr);
+    s->slot_chunk.assign(nslots, -1);
+    s->slot_used.assign(nslots, 0);
+    s->chunk_slot.assign(C, -1);
+    s->slot_free.assign(nslots, nullptr);
+    s->slot_ready.assign(nslots, nullptr);
+    for (uint64_t i = 0; i < nslots; i++) {
+        if ((rc = galloc(g, (void **)&s->slot[i], slot_bytes))) return rc;
+        if (hipEventCreateWithFlags(&s->slot_free[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->slot_ready[i], hipEventDisableTiming) != hipSuccess)
+            return fail(PTG_ERR_HIP, "hipEventCreate");
+        PTG_HIP(hipEventRecord(s->slot_free[i], g->stream));
+    }
+    if (hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
+        return fail(PTG_ERR_HIP, "copy stream");
+    return PTG_OK;
+}
+
+void destroy_streamed(ptg_graph *g) {
+    StreamState *s = g->streamed;
+    if (!s) return;
+    if (s->copy) {
+        (void)hipStreamSynchronize(s->copy);
+        (void)hipStreamDestroy(s->copy);
+    }
+    for (auto *p : s->slot) gfree(g, p);
+    for (auto e : s->slot_free)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : s->slot_ready)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : s->tcopy) (void)hipEventDestroy(e);
+    for (auto *p : {s->d_pv, s->d_pe0, s->d_pe1, s->d_rv, s->d_rpb, s->d_rpe, s->d_cvb, s->d_cve, s->d_need,
+                    s->d_bits[0], s->d_bits[1]})
+        gfree(g, p);
+    gfree(g, s->d_cnt);
+    gfree(g, s->d_contrib[0]);
+    gfree(g, s->d_contrib[1]);
+    gfree(g, s->d_ppart);
+    gfree(g, s->d_dpart);
+    gfree(g, s->d_base);
+    if (s->h_col) (void)hipHostFree(s->h_col);
+    if (s->h_need) (void)hipHostFree(s->h_need);
+    if (s->h_cnt) (void)hipHostFree(s->h_cnt);
+    delete s;
+    g->streamed = nullptr;
+}
+
+// Makes chunk `c` resident and returns its slot; the graph's stream waits for
+// the copy. `pending[c']` marks chunks still to run in this step (kept if
+// another slot can be freed).
+static int acquire_chunk(ptg_graph *g, uint32_t c, const std::vector<char> &pending, uint32_t *slot_out) {
+    StreamState *s = g->streamed;
+    s->clock++;
+    if (s->chunk_slot[c] >= 0) {
+        const uint32_t sl = (uint32_t)s->chunk_slot[c];
+        s->slot_used[sl] = s->clock;
+        s->hits++;
+        *slot_out = sl;
+        return PTG_OK;
+    }
+    int best = -1;
+    uint64_t best_key = ~0ull;
+    for (size_t i = 0; i < s->slot.size(); i++) {
+        const int held = s->slot_chunk[i];
+        const uint64_t key = held < 0 ? 0 : (pending[held] ? (1ull << 62) + s->slot_used[i] : s->slot_used[i]);
+        if (key < best_key) {
+            best_key = key;
+            best = (int)i;
+        }
+    }
+    const uint32_t sl = (uint32_t)best;
+    if (s->slot_chunk[sl] >= 0) s->chunk_slot[s->slot_chunk[sl]] = -1;
+    s->slot_chunk[sl] = (int)c;
+    s->chunk_slot[c] = (int)sl;
+    s->slot_used[sl] = s->clock;
+    const ChunkDesc &ch = s->chunks[c];
+    const uint64_t bytes = 4 * (ch.ee - ch.eb);
+    if (bytes) {
+        // the copy waits until the kernels that read the slot's previous chunk are done
+        PTG_HIP(hipStreamWaitEvent(s->copy, s->slot_free[sl], 0));
+        if (s->tcopy.size() < 2ull * (s->tcopy_used + 1)) {
+            hipEvent_t a, b;
+            PTG_HIP(hipEventCreate(&a));
+            PTG_HIP(hipEventCreate(&b));
+            s->tcopy.push_back(a);
+            s->tcopy.push_back(b);
+        }
+        PTG_HIP(hipEventRecord(s->tcopy[2 * s->tcopy_used], s->copy));
+        PTG_HIP(hipMemcpyAsync(s->slot[sl], s->h_col + ch.eb, bytes, hipMemcpyHostToDevice, s->copy));
+        PTG_HIP(hipEventRecord(s->tcopy[2 * s->tcopy_used + 1], s->copy));
+        s->tcopy_used++;
+        PTG_HIP(hipEventRecord(s->slot_ready[sl], s->copy));
+        PTG_HIP(hipStreamWaitEvent(g->stream, s->slot_ready[sl], 0));
+        s->bytes += bytes;
+    }
+    s->loads++;
+    *slot_out = sl;
+    return PTG_OK;
+}
+
+static ChunkArgs chunk_args(const StreamState *s, uint32_t c, uint32_t sl) {
+    const ChunkDesc &ch = s->chunks[c];
+    ChunkArgs a;
+    a.eb = ch.eb;
+    a.vh = ch.vh;
+    a.ve = ch.ve;
+    a.pb = ch.pb;
+    a.pe = ch.pe;
+    a.slot = s->slot[sl];
+    return a;
+}
+
+static void begin_query(StreamState *s) {
+    s->tcopy_used = 0;
+    s->loads = s->hits = s->bytes = 0;
+}
+
+static int finish_query_stats(ptg_graph *g, ptg_stats *st) {
+    StreamState *s = g->streamed;
+    PTG_HIP(hipStreamSynchronize(s->copy));
+    double ns = 0;
+    for (uint32_t i = 0; i < s->tcopy_used; i++) {
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, s->tcopy[2 * i], s->tcopy[2 * i + 1]));
+        ns += (double)ms * 1e6;
+    }
+    st->host_link_bytes = s->bytes;
+    st->host_link_ns = (uint64_t)ns;
+    st->chunk_loads = (uint32_t)s->loads;
+    st->chunk_hits = (uint32_t)s->hits;
+    return PTG_OK;
+}
+
+static int sum_counter(ptg_graph *g, uint64_t *v) {
+    StreamState *s = g->streamed;
+    PTG_HIP(hipMemcpyAsync(s->h_cnt, s->d_cnt, 16 * kShards, hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    uint64_t t = 0;
+    for (uint32_t i = 0; i < 2 * kShards; i++) t += s->h_cnt[i];
+    *v = t;
+    return PTG_OK;
+}
+
+int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
+    StreamState *s = g->streamed;
+    const uint32_t V = g->V;
+    const uint32_t C = (uint32_t)s->chunks.size();
+    const uint32_t nwords = (V + 31) / 32;
+    int rc;
+    begin_query(s);
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if ((rc = fill_u32(g, level, V, 0xFFFFFFFFu)) || (rc = fill_u32(g, s->d_bits[0], nwords, 0u)) ||
+        (rc = fill_u32(g, s->d_bits[1], nwords, 0u)))
+        return rc;
+    k_stream_seed<<<1, 64, 0, g->stream>>>(level, s->d_bits[0], src);
+    PTG_HIP(hipGetLastError());
+    uint32_t iters = 0;
+    uint64_t fsum = 1;
+    std::vector<char> pending(C, 0);
+    for (uint32_t L = 0;; L++) {
+        uint32_t *cur = s->d_bits[L & 1], *nxt = s->d_bits[(L + 1) & 1];
+        // which chunks hold frontier vertices
+        if ((rc = fill_u32(g, s->d_need, C, 0u)) ||
+            (rc = fill_u32(g, reinterpret_cast<uint32_t *>(s->d_cnt), 4 * kShards, 0u)))
+            return rc;
+        const uint32_t nb = std::min<uint32_t>((nwords + kBlock - 1) / kBlock, g->grid);
+        k_stream_need<<<std::max(nb, 1u), kBlock, 0, g->stream>>>(nwords, cur, C, s->d_cvb, s->d_cve, s->d_need,
+                                                                   s->d_cnt);
+        PTG_HIP(hipGetLastError());
+        PTG_HIP(hipMemcpyAsync(s->h_need, s->d_need, 4ull * C, hipMemcpyDeviceToHost, g->stream));
+        PTG_HIP(hipStreamSynchronize(g->stream));
+        std::vector<uint32_t> order;
+        for (uint32_t c = 0; c < C; c++)
+            if (s->h_need[c] && s->chunk_slot[c] >= 0) order.push_back(c);
+        for (uint32_t c = 0; c < C; c++)
+            if (s->h_need[c] && s->chunk_slot[c] < 0) order.push_back(c);
+        if (order.empty()) break;
+        for (uint32_t c : order) pending[c] = 1;
+        if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(s->d_cnt), 4 * kShards, 0u))) return rc;
+        for (uint32_t c : order) {
+            uint32_t sl = 0;
+            if ((rc = acquire_chunk(g, c, pending, &sl))) return rc;
+            pending[c] = 0;
+            k_stream_bfs<<<kStreamGrid, kBlock, 0, g->stream>>>(chunk_args(s, c, sl), g->out.off, s->d_pv, s->d_pe0,
+                                                                s->d_pe1, level, cur, nxt, L + 2, s->d_cnt);
+            PTG_HIP(hipGetLastError());
+            PTG_HIP(hipEventRecord(s->slot_free[sl], g->stream));
+        }
+        uint64_t found = 0;
+        if ((rc = sum_counter(g, &found))) return rc;
+        iters++;
+        if ((rc = fill_u32(g, cur, nwords, 0u))) return rc;
+        if (!found) break;
+        fsum += found;
+    }
+    PTG_HIP(hipEventRecord(g->ev1, g->stream));
+    PTG_HIP(hipEventSynchronize(g->ev1));
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        st->kernel_ns = (uint64_t)((double)ms * 1e6);
+        st->iterations = iters;
+        st->frontier_sum = fsum;
+        if ((rc = finish_query_stats(g, st))) return rc;
+        if ((rc = reach_stats(g, level, 0xFFFFFFFFu, &st->reached, &st->edges_traversed))) return rc;
+    }
+    return PTG_OK;
+}
+
+int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
+    StreamState *s = g->streamed;
+    if (!g->symmetric) return fail(PTG_ERR_UNSUPPORTED, "streamed PageRank needs a symmetric graph");
+    const uint32_t V = g->V;
+    const uint32_t C = (uint32_t)s->chunks.size();
+    const float r0 = 1.0f / (float)V;
+    int rc;
+    begin_query(s);
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if (iters == 0) {
+        union {
+            float f;
+            uint32_t u;
+        } cv;
+        cv.f = r0;
+        if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(ranks), V, cv.u))) return rc;
+    } else {
+        k_stream_pr_init<<<std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid), kBlock, 0, g->stream>>>(
+            V, g->out.off, r0, s->d_contrib[0]);
+        // D0 = (vertices without out-edges) / V as (hi, lo)
+        const double d0v = (double)s->zeros / (double)V;
+        const float d0[2] = {(float)d0v, (float)(d0v - (double)(float)d0v)};
+        if ((rc = copy_sync(g, s->d_base + 2, d0, 8, hipMemcpyHostToDevice))) return rc;
+        std::vector<char> pending(C, 0);
+        const uint64_t slots_per_iter = (uint64_t)C * kStreamGrid;
+        for (int t = 0; t < iters; t++) {
+            float *dprev = s->d_dpart + 2 * slots_per_iter * ((t + 1) & 1);
+            float *dcur = s->d_dpart + 2 * slots_per_iter * (t & 1);
+            k_stream_pr_base<<<1, kBlock, 0, g->stream>>>(dprev, (uint32_t)slots_per_iter, t == 0 ? s->d_base + 2 : nullptr,
+                                                         damping, r0, s->d_base);
+            PrStep p{g->out.off, s->d_contrib[t & 1], s->d_contrib[(t + 1) & 1], t == iters - 1 ? ranks : nullptr,
+                     s->d_base, damping};
+            // alternate the order so the chunks left resident by one pass start the next
+            for (uint32_t i = 0; i < C; i++) pending[i] = 1;
+            for (uint32_t i = 0; i < C; i++) {
+                const uint32_t c = (t & 1) ? C - 1 - i : i;
+                uint32_t sl = 0;
+                if ((rc = acquire_chunk(g, c, pending, &sl))) return rc;
+                pending[c] = 0;
+                k_stream_pr<<<kStreamGrid, kBlock, 0, g->stream>>>(chunk_args(s, c, sl), p, s->d_pe0, s->d_pe1,
+                                                                   s->d_ppart, dcur + 2ull * c * kStreamGrid);
+                const ChunkDesc &ch = s->chunks[c];
+                if (ch.re > ch.rb)
+                    k_stream_pr_hubs<<<std::min<uint32_t>((ch.re - ch.rb + 3) / 4, kStreamGrid), kBlock, 0,
+                                       g->stream>>>(ch.rb, ch.re, s->d_rv, s->d_rpb, s->d_rpe, s->d_ppart, p);
+                PTG_HIP(hipGetLastError());
+                PTG_HIP(hipEventRecord(s->slot_free[sl], g->stream));
+            }
+            if (s->nruns > s->split_rb)
+                k_stream_pr_hubs<<<std::min<uint32_t>((s->nruns - s->split_rb + 3) / 4, kStreamGrid), kBlock, 0,
+                                   g->stream>>>(s->split_rb, s->nruns, s->d_rv, s->d_rpb, s->d_rpe, s->d_ppart, p);
+            PTG_HIP(hipGetLastError());
+        }
+    }
+    PTG_HIP(hipEventRecord(g->ev1, g->stream));
+    PTG_HIP(hipEventSynchronize(g->ev1));
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        st->kernel_ns = (uint64_t)((double)ms * 1e6);
+        st->iterations = (uint32_t)iters;
+        st->reached = V;
+        st->edges_traversed = g->E * (uint64_t)iters;
+        if ((rc = finish_query_stats(g, st))) return rc;
+    }
+    return PTG_OK;
+}
+
+int stream_info(const ptg_graph *g, uint32_t *chunks, uint32_t *slots, uint64_t *slot_bytes) {
+    const StreamState *s = g->streamed;
+    uint64_t largest = 1;
+    for (auto &ch : s->chunks) largest = std::max<uint64_t>(largest, ch.ee - ch.eb);
+    if (chunks) *chunks = (uint32_t)s->chunks.size();
+    if (slots) *slots = (uint32_t)s->slot.size();
+    if (slot_bytes) *slot_bytes = al256(4 * largest);
+    return PTG_OK;
+}
+
+// Pinned host -> device copy rate: `reps` copies of `bytes` (timed with events).
+int host_link_probe(uint64_t bytes, int reps, double *gbs) {
+    void *h = nullptr, *d = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = PTG_OK;
+    if (hipHostMalloc(&h, bytes, hipHostMallocDefault) != hipSuccess) return fail(PTG_ERR_OUT_OF_MEMORY, "pinned probe");
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail(PTG_ERR_OUT_OF_MEMORY, "device probe");
+    }
+    memset(h, 1, bytes);
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
+        hipEventCreate(&b) != hipSuccess) {
+        rc = fail(PTG_ERR_HIP, "probe stream/events");
+    } else {
+        (void)hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st);  // warm
+        (void)hipEventRecord(a, st);
+        for (int i = 0; i < reps; i++) (void)hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st);
+        (void)hipEventRecord(b, st);
+        if (hipEventSynchronize(b) != hipSuccess) {
+            rc = fail(PTG_ERR_HIP, "probe copy");
+        } else {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            *gbs = (double)bytes * reps / ((double)ms * 1e6);
+        }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(d);
+    (void)hipHostFree(h);
+    return rc;
+}
+
+}  // namespace ptg

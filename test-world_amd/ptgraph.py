@@ -29,6 +29,7 @@ ERR_IO = -9
 FLAG_NO_VALIDATE = 0x1
 FLAG_TIMING = 0x2
 FLAG_NO_RELABEL = 0x4
+FLAG_STREAM = 0x8
 TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH = 0x1, 0x2, 0x4, 0x8
 Q_BFS, Q_SSSP, Q_CC, Q_PAGERANK = 0, 1, 2, 3
 
@@ -86,7 +87,9 @@ class Stats(ctypes.Structure):
         ("host_link_ns", ctypes.c_uint64),
         ("main_kernel_ns", ctypes.c_uint64),
         ("main_launches", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("chunk_loads", ctypes.c_uint32),
+        ("chunk_hits", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -170,6 +173,9 @@ SIGNATURES = {
     "ptg_device_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "ptg_device_free": (ctypes.c_int, [_vp]),
     "ptg_memcpy_d2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    "ptg_graph_stream_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                             ctypes.POINTER(ctypes.c_uint64)]),
+    "ptg_host_link_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "ptg_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "ptg_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "ptg_validate_csr": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p,
@@ -268,6 +274,13 @@ def default_config(**overrides):
     for k, v in overrides.items():
         setattr(c, k, v)
     return c
+
+
+def host_link_probe(nbytes=256 << 20, reps=8):
+    """Pinned host -> device copy rate in GB/s."""
+    x = ctypes.c_double(0)
+    _check(lib().ptg_host_link_probe(nbytes, reps, ctypes.byref(x)), "ptg_host_link_probe")
+    return x.value
 
 
 def device_info():
@@ -509,6 +522,13 @@ class Graph:
     def set_flags(self, flags):
         """Changes runtime flags (only FLAG_TIMING may change after creation)."""
         _check(lib().ptg_graph_set_flags(self._h, flags), "ptg_graph_set_flags")
+
+    def stream_info(self):
+        """(chunks, device slots, bytes per slot) of a PTG_FLAG_STREAM graph."""
+        c, n, b = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint64(0)
+        _check(lib().ptg_graph_stream_info(self._h, ctypes.byref(c), ctypes.byref(n), ctypes.byref(b)),
+               "ptg_graph_stream_info")
+        return c.value, n.value, b.value
 
     def get_config(self):
         c = Config()

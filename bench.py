@@ -26,6 +26,9 @@ same 8 sources (16 queries), as GTEPS over the same m_r.
 After the timed steps (none of this is in `value`):
   batched  64 sources (seed 3) per launch on the same graph, batch knobs
            picked by the autotuner, vs the same 64 single queries: queries/s.
+  partition  R-MAT scale 26 with its edges in pinned host memory, streamed
+           in 256 MB chunks into a device cache under a 4 GB budget: BFS and
+           PageRank(10), host-link GB/s against a pinned copy probe.
   serving  6 resident R-MAT graphs (scales 18..24), 3000 requests, mix 50%
            BFS / 30% SSSP / 15% CC / 5% PageRank(10), Poisson arrivals at
            400/s (seed 4): the plain per-query path (one at a time) vs the
@@ -173,6 +176,55 @@ def serving_leg(args, pg, np, cores, log):
     return out
 
 
+def partition_leg(args, pg, np, cores, log):
+    """Edges kept in pinned host memory and streamed in 256 MB chunks into a
+    device cache under a device budget (R-MAT scale 26 under 4 GB by
+    default): BFS from vertex 0 (3 runs) and PageRank (10 iterations), with
+    the host-link rate against a pinned copy probe and the bytes moved
+    against the lower bound (4 bytes per column of every reached vertex for
+    BFS; the non-resident columns, every iteration, for PageRank)."""
+    sc = args.partition_scale
+    t = time.time()
+    csr = pg.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=1, threads=cores)
+    V, E = csr.V, csr.E
+    budget = int(args.partition_budget_gb * (1 << 30))
+    cfg = pg.default_config(flags=pg.FLAG_STREAM, chunk_bytes=256 << 20, device_budget_bytes=budget)
+    g = pg.Graph(csr.node_pointers, csr.cols, config=cfg)
+    del csr
+    chunks, slots, slot_bytes = g.stream_info()
+    log("partition: rmat%d V=%d E=%d, %d chunks, %d slots, device %.2f GB, ready in %.1fs" % (
+        sc, V, E, chunks, slots, g.device_bytes / 1e9, time.time() - t))
+    probe = pg.host_link_probe(256 << 20, 8)
+    out = pg.DeviceBuffer(4 * V)
+    res = {"graph": "rmat%d" % sc, "V": V, "E": E, "device_budget_bytes": budget, "device_bytes": g.device_bytes,
+           "chunks": chunks, "slots": slots, "chunk_bytes": slot_bytes, "host_link_probe_gbs": round(probe, 2)}
+    try:
+        runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            _, st = g.bfs(0, out=out)
+            dt = time.perf_counter() - t
+            runs.append((dt, st.edges_traversed, st.host_link_bytes, st.host_link_ns, st.chunk_loads, st.chunk_hits))
+        dt, m_r, lb, lns, loads, hits = min(runs)
+        res["bfs"] = {"ms": round(dt * 1e3, 1), "gteps": round(m_r / dt / 1e9, 3),
+                      "link_bytes": lb, "link_gbs": round(lb / max(lns, 1), 2),
+                      "link_vs_lower_bound": round(lb / (4.0 * m_r), 3), "chunk_loads": loads, "chunk_hits": hits}
+        t = time.perf_counter()
+        _, st = g.pagerank(0.85, 10, out=out)
+        dt = time.perf_counter() - t
+        lower = 10 * max(0, 4 * E - slots * slot_bytes)
+        res["pagerank10"] = {"ms": round(dt * 1e3, 1), "iters_per_s": round(10 / dt, 2),
+                             "link_bytes": st.host_link_bytes,
+                             "link_gbs": round(st.host_link_bytes / max(st.host_link_ns, 1), 2),
+                             "link_vs_lower_bound": round(st.host_link_bytes / max(lower, 1), 3),
+                             "chunk_loads": st.chunk_loads, "chunk_hits": st.chunk_hits}
+    finally:
+        out.free()
+        g.close()
+    log("partition: %s" % res)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +239,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-source leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tuning (no autotuner run)")
+    ap.add_argument("--no-partition", action="store_true", help="skip the streamed (partition) leg")
+    ap.add_argument("--partition-scale", type=int, default=26)
+    ap.add_argument("--partition-budget-gb", type=float, default=4.0)
     ap.add_argument("--serve-scales", default="18,20,21,22,23,24")
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--serve-requests", type=int, default=3000)
@@ -295,6 +350,7 @@ def main():
         log("autotuner %.1fs: %s" % (time.time() - t, tuned))
     batched = None if args.no_batch else batched_leg(g, srcs64, V, pg, log)
     serving = None if args.no_serve else serving_leg(args, pg, np, cores, log)
+    partition = None if args.no_partition else partition_leg(args, pg, np, cores, log)
 
     def tot(k, idx):
         return sum(r[idx] for r in acc[k])
@@ -409,6 +465,7 @@ def main():
         "tuning": tuned,
         "batched": batched,
         "serving": serving,
+        "partition": partition,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -50,7 +50,9 @@ struct RoundCtr {
     uint32_t any;                           // 1 if the producing round emitted anything
     unsigned long long seen;                // BFS: sum of out-degrees of earlier frontiers
     uint32_t far_any;                       // SSSP: far set non-empty after the producing round
-    uint32_t pad[9];
+    uint32_t need_heavy;                    // SSSP: the bucket's heavy edges are still to relax
+    uint32_t lo;                            // SSSP: lower end of the current bucket
+    uint32_t pad[7];
     uint32_t found_sh[kShards];             // frontier vertices (any degree)
     uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
     unsigned long long edges_sh[kShards];   // sum of out-degrees of the frontier
@@ -166,6 +168,12 @@ struct ptg_graph {
     uint32_t *d_inv = nullptr;          // new id -> old id
     uint32_t *d_val2 = nullptr;         // V: results mapped back to old ids
 
+    // SSSP light/heavy split of the weighted CSR for step split_delta
+    // (sssp.hip): light = edges of weight < delta, heavy = the rest.
+    uint32_t split_delta = 0;
+    ptg::Csr light, heavy;
+    uint32_t *d_wl = nullptr, *d_wh = nullptr;
+
     // Batched multi-source state (batch.hip), allocated on the first batch.
     ptg::BatchState *batch = nullptr;
     // PTG_FLAG_STREAM: host-resident edges and the device chunk cache (stream.hip).
@@ -215,6 +223,8 @@ void destroy_batch(ptg_graph *g);
 // lazily allocated query state, and per-row checksums of n x V results.
 uint64_t batch_state_need(const ptg_graph *g, bool sssp);
 uint64_t pr_state_need(const ptg_graph *g);
+uint64_t sssp_split_need(const ptg_graph *g);
+void release_sssp_split(ptg_graph *g);
 void release_pr_state(ptg_graph *g);
 int row_checksums(ptg_graph *g, hipStream_t stream, const uint32_t *rows, uint32_t n, unsigned long long *d_out);
 // Streamed graphs (stream.hip).

@@ -282,10 +282,14 @@ __device__ __forceinline__ void expand_frontier(const RoundEnv &e, const RoundCt
 // ---- host side ---------------------------------------------------------------
 
 RoundEnv make_env(ptg_graph *g, uint32_t r);
+// Same over another CSR with the graph's vertex set (SSSP's light edges).
+RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint32_t *col);
 // Clears the frontier bitmaps, byte flags and counter ring before a search.
 int reset_rounds(ptg_graph *g);
 // Seeds round 0 with one source of value src_value (queue mode).
 int launch_seed(ptg_graph *g, uint32_t src, uint32_t src_value, uint32_t *val, uint32_t thr);
+int launch_seed_csr(ptg_graph *g, const uint32_t *off, const uint32_t *col, uint32_t src, uint32_t src_value,
+                    uint32_t *val, uint32_t thr);
 // Pre-pass of round r (a no-op for queue-mode frontiers).
 int launch_prepare(ptg_graph *g, const RoundEnv &e);
 int read_ring(ptg_graph *g, uint32_t from, uint32_t count);

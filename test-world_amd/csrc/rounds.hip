@@ -7,12 +7,14 @@
 
 namespace ptg {
 
-RoundEnv make_env(ptg_graph *g, uint32_t r) {
+RoundEnv make_env(ptg_graph *g, uint32_t r) { return make_env_csr(g, r, g->out.off, g->out.col); }
+
+RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint32_t *col) {
     RoundEnv e;
     e.ring = g->d_ring;
     e.r = r;
-    e.off = g->out.off;
-    e.col = g->out.col;
+    e.off = off;
+    e.col = col;
     e.fin = g->d_bits[r % 3];
     e.fout = g->d_bits[(r + 1) % 3];
     e.fclr = g->d_bits[(r + 2) % 3];
@@ -139,6 +141,7 @@ __global__ void k_seed(RoundEnv e, uint32_t src, uint32_t src_value, uint32_t *v
     c->mode = kModeQueue;
     c->dir = kDirTopDown;
     c->thr = thr;
+    c->lo = src_value;
     c->any = 1u;
     c->found_sh[0] = 1;
     c->edges_sh[0] = d;
@@ -159,6 +162,13 @@ int reset_rounds(ptg_graph *g) {
     for (int i = 0; i < 2; i++)
         if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_flags[i]), (uint64_t)g->npairs * 16, 0u))) return rc;
     return fill_u32(g, reinterpret_cast<uint32_t *>(g->d_ring), sizeof(RoundCtr) / 4 * kRing, 0u);
+}
+
+int launch_seed_csr(ptg_graph *g, const uint32_t *off, const uint32_t *col, uint32_t src, uint32_t src_value,
+                    uint32_t *val, uint32_t thr) {
+    k_seed<<<1, 64, 0, g->stream>>>(make_env_csr(g, 0, off, col), src, src_value, val, thr);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
 }
 
 int launch_seed(ptg_graph *g, uint32_t src, uint32_t src_value, uint32_t *val, uint32_t thr) {

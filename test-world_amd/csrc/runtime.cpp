@@ -214,6 +214,7 @@ static void destroy(ptg_graph *g) {
     for (auto *c : g->d_contrib) gfree(g, c);
     destroy_batch(g);
     destroy_streamed(g);
+    release_sssp_split(g);
     gfree(g, g->d_part);
     gfree(g, g->d_lpart);
     if (g->h_ring) (void)hipHostFree(g->h_ring);

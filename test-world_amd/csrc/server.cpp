@@ -89,6 +89,7 @@ uint64_t resident(const ptg_server *s) {
 void release_state(Slot *sl) {
     destroy_batch(sl->g);
     release_pr_state(sl->g);
+    release_sssp_split(sl->g);
     sl->held = sl->g->device_bytes - sl->base_bytes;
 }
 
@@ -147,6 +148,7 @@ std::vector<Pending> form_launch(ptg_server *s, Slot *sl) {
 uint64_t launch_need(const Slot *sl, uint32_t kind, size_t n) {
     if ((kind == PTG_Q_BFS || kind == PTG_Q_SSSP) && n > 1) return batch_state_need(sl->g, kind == PTG_Q_SSSP);
     if (kind == PTG_Q_PAGERANK) return pr_state_need(sl->g);
+    if (kind == PTG_Q_SSSP) return sssp_split_need(sl->g);
     return 0;
 }
 

@@ -28,6 +28,8 @@ CONFIGS = {
     "odd_grid": {"grid_blocks": 7, "small_degree": 3, "large_degree": 64},
     "no_relabel": {"flags": pg.FLAG_NO_RELABEL},
     "no_relabel_bu": {"flags": pg.FLAG_NO_RELABEL, "td_to_bu": 1e9, "bu_to_td": 1e9, "small_degree": 2},
+    "delta1": {"sssp_delta": 1},      # light = zero-weight edges only
+    "delta1000": {"sssp_delta": 1000},  # every edge light
 }
 
 
@@ -288,3 +290,21 @@ def test_set_flags_timing(rmat16):
     with pytest.raises(pg.PtgError) as e:
         gg.set_flags(pg.FLAG_NO_RELABEL)
     assert e.value.code == pg.ERR_INVALID_ARG
+
+
+def test_sssp_steps_rmat16(rmat16):
+    """Light/heavy rounds for several steps (the split CSR is rebuilt per step)."""
+    g, gg = rmat16
+    ptr, col, w = g.node_pointers, g.cols, g.weights
+    deg = np.diff(g.offsets.astype(np.int64))
+    srcs = [0] + [int(x) for x in np.random.default_rng(9).choice(np.nonzero(deg > 0)[0], 3, replace=False)]
+    want = {s: oracle.dijkstra(ptr, col, w, s) for s in srcs}
+    try:
+        for delta in (1, 4, 16, 63, 64, 65, 4096):
+            gg.tune(sssp_delta=delta)
+            say("case rmat16 sssp delta %d" % delta)
+            for s in srcs:
+                d, st = gg.sssp(s)
+                np.testing.assert_array_equal(d, want[s], err_msg="delta %d src %d" % (delta, s))
+    finally:
+        gg.tune(sssp_delta=0)

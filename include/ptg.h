@@ -86,6 +86,9 @@ typedef struct ptg_config {
     uint32_t batch_grid;          /* workgroups of the batched kernels; 0 = grid_blocks */
     float batch_td_to_bu;         /* batched BFS goes bottom-up when m_f > m_u / this; 0 = td_to_bu */
     uint32_t batch_chunk;         /* edges per static chunk of batched SSSP; 0 = auto (256) */
+    uint32_t order_hub_degree;    /* vertex order on the device: 0 = by descending degree; d > 0 =
+                                     vertices of degree > d by degree, then breadth-first from them */
+    uint32_t reserved;
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,

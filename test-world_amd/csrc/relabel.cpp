@@ -25,6 +25,65 @@ static int worker_count() {
     return (int)std::min<unsigned>(h ? h : 1, 32);
 }
 
+// Rewrites every adjacency list in new ids (sorted ascending, weights carried
+// along) once perm / inv are set.
+static void rebuild(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
+                    const EdgeWithWeight *wedges, Relabeled *out);
+
+// Hubs (out-degree > hub_degree) first by descending degree, then every other
+// vertex in the order a breadth-first search from the hubs reaches it (so the
+// neighbours of one hub get neighbouring ids and a hub's expansion gathers
+// from a few cache lines), other components after, vertices without edges
+// last.
+static void order_hubs_bfs(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols,
+                           const EdgeWithWeight *wedges, uint32_t hub_degree, Relabeled *out) {
+    auto deg = [&](uint32_t v) -> uint32_t { return (v + 1 < V ? ptr[v + 1] : (uint32_t)E) - ptr[v]; };
+    auto col = [&](uint64_t k) -> uint32_t { return cols ? cols[k] : wedges[k].toNode; };
+    std::vector<uint32_t> order;
+    order.reserve(V);
+    std::vector<uint8_t> seen(V, 0);
+    for (uint32_t v = 0; v < V; v++)
+        if (deg(v) > hub_degree) order.push_back(v);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return deg(a) > deg(b); });
+    for (uint32_t h : order) seen[h] = 1;
+    size_t head = 0;
+    auto bfs = [&] {
+        while (head < order.size()) {
+            const uint32_t u = order[head++];
+            const uint64_t b = ptr[u], e = b + deg(u);
+            for (uint64_t k = b; k < e; k++) {
+                const uint32_t w = col(k);
+                if (!seen[w] && deg(w) > 0) {
+                    seen[w] = 1;
+                    order.push_back(w);
+                }
+            }
+        }
+    };
+    bfs();
+    for (uint32_t v = 0; v < V; v++)
+        if (!seen[v] && deg(v) > 0) {
+            seen[v] = 1;
+            order.push_back(v);
+            bfs();
+        }
+    for (uint32_t v = 0; v < V; v++)
+        if (deg(v) == 0) order.push_back(v);
+    out->perm.resize(V);
+    out->inv.swap(order);
+    for (uint32_t i = 0; i < V; i++) out->perm[out->inv[i]] = i;
+}
+
+void relabel(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
+             const EdgeWithWeight *wedges, uint32_t hub_degree, Relabeled *out) {
+    if (hub_degree == 0) {
+        relabel_by_degree(V, E, ptr, cols, weights, wedges, out);
+        return;
+    }
+    order_hubs_bfs(V, E, ptr, cols, wedges, hub_degree, out);
+    rebuild(V, E, ptr, cols, weights, wedges, out);
+}
+
 void relabel_by_degree(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
                        const EdgeWithWeight *wedges, Relabeled *out) {
     auto deg = [&](uint32_t v) -> uint32_t { return (v + 1 < V ? ptr[v + 1] : (uint32_t)E) - ptr[v]; };
@@ -41,6 +100,12 @@ void relabel_by_degree(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
         out->perm[v] = n;
         out->inv[n] = v;
     }
+    rebuild(V, E, ptr, cols, weights, wedges, out);
+}
+
+static void rebuild(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
+                    const EdgeWithWeight *wedges, Relabeled *out) {
+    auto deg = [&](uint32_t v) -> uint32_t { return (v + 1 < V ? ptr[v + 1] : (uint32_t)E) - ptr[v]; };
     out->off.resize((size_t)V + 1);
     out->off[0] = 0;
     for (uint32_t i = 0; i < V; i++) out->off[i + 1] = out->off[i] + deg(out->inv[i]);

@@ -21,5 +21,10 @@ struct Relabeled {
 // Exactly one of cols / wedges is used; weights may be null.
 void relabel_by_degree(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
                        const EdgeWithWeight *wedges, Relabeled *out);
+// hub_degree == 0: by degree as above; otherwise vertices of out-degree >
+// hub_degree first by degree, then the others in breadth-first order from
+// them, vertices without out-edges last.
+void relabel(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
+             const EdgeWithWeight *wedges, uint32_t hub_degree, Relabeled *out);
 
 }  // namespace ptg

@@ -439,7 +439,7 @@ int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, con
         col = tmp.data();
     }
     if (!(c.flags & PTG_FLAG_NO_RELABEL)) {
-        relabel_by_degree(V, E, ptr, col, nullptr, nullptr, &rl);
+        relabel(V, E, ptr, col, nullptr, nullptr, c.order_hub_degree, &rl);
         off.swap(rl.off);
         col = rl.col.data();
         g->relabeled = true;

@@ -73,6 +73,8 @@ class Config(ctypes.Structure):
         ("batch_grid", ctypes.c_uint32),
         ("batch_td_to_bu", ctypes.c_float),
         ("batch_chunk", ctypes.c_uint32),
+        ("order_hub_degree", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

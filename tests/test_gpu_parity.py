@@ -28,6 +28,8 @@ CONFIGS = {
     "odd_grid": {"grid_blocks": 7, "small_degree": 3, "large_degree": 64},
     "no_relabel": {"flags": pg.FLAG_NO_RELABEL},
     "no_relabel_bu": {"flags": pg.FLAG_NO_RELABEL, "td_to_bu": 1e9, "bu_to_td": 1e9, "small_degree": 2},
+    "degree_order": {"order_hub_degree": 0},
+    "hub_order_big": {"order_hub_degree": 3},
     "delta1": {"sssp_delta": 1},      # light = zero-weight edges only
     "delta1000": {"sssp_delta": 1000},  # every edge light
 }

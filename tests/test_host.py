@@ -81,6 +81,7 @@ def test_config_defaults():
     assert c.sssp_sentinel == 0xFFFFFFFE
     assert c.small_degree < c.large_degree
     assert c.chunk_bytes == 256 << 20
+    assert c.order_hub_degree == 16
 
 
 def check_clean_symmetric(g):

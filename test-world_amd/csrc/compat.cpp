@@ -12,12 +12,25 @@
 
 namespace {
 
+// `may_stream`: the query runs on a streamed graph too (BFS, PageRank on a
+// symmetric graph); when the resident graph would not fit the device's free
+// memory, such a driver streams the edges like the reference's partition
+// drivers (bfsCaculateInOpt / bfsCaculateInAsyncSwap, main.cu:1180-2059).
 template <class F>
 long run_one(uint32_t V, uint32_t E, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges,
-             uint32_t sentinel, F query) {
+             uint32_t sentinel, bool may_stream, F query) {
     ptg_config cfg;
     ptg_config_default(&cfg);
     cfg.sssp_sentinel = sentinel;
+    if (may_stream && cols) {
+        int cus = 0;
+        uint64_t total = 0, free_b = 0;
+        const uint64_t need = ptg_graph_estimate_bytes(V, E, 0, &cfg);
+        if (ptg_device_info(&cus, &total, &free_b) == PTG_OK && need > free_b / 10 * 9) {
+            cfg.flags |= PTG_FLAG_STREAM;
+            cfg.device_budget_bytes = free_b / 10 * 9;
+        }
+    }
     ptg_graph *g = nullptr;
     int rc = ptg_graph_create(V, E, ptr, cols, wedges, &cfg, &g);
     if (rc) return rc;
@@ -35,7 +48,7 @@ long run_one(uint32_t V, uint32_t E, const uint32_t *ptr, const uint32_t *cols, 
 extern "C" {
 
 long bfsCaculateInShare(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edgeList, uint32_t src) {
-    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu,
+    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu, true,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_bfs(g, src, o, nullptr); });
 }
 
@@ -53,18 +66,18 @@ long bfsCaculateInAsyncSwap(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edg
 
 long ssspCaculateInShare(uint32_t V, uint32_t E, uint32_t *ptr, EdgeWithWeight *edgeList, uint32_t src) {
     // unreached = UINT_MAX - 1 (main.cu:584)
-    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFEu,
+    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFEu, false,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_sssp(g, src, o, nullptr); });
 }
 
 long ssspCaculateInOpt(uint32_t V, uint32_t E, uint32_t *ptr, EdgeWithWeight *edgeList, uint32_t src) {
     // unreached = UINT_MAX (main.cu:742)
-    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFFu,
+    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFFu, false,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_sssp(g, src, o, nullptr); });
 }
 
 long ccCaculateInShare(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edgeList) {
-    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu,
+    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu, false,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_cc(g, o, nullptr); });
 }
 
@@ -75,7 +88,7 @@ long ccCaculateInOpt(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edgeList) 
 // The reference's PageRank drivers compute nothing; these run the defined
 // PageRank (damping 0.85, 20 iterations).
 long prCaculateInShare(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edgeList) {
-    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu, [&](ptg_graph *g, uint32_t *o) {
+    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu, true, [&](ptg_graph *g, uint32_t *o) {
         return ptg_pagerank(g, 0.85f, 20, reinterpret_cast<float *>(o), nullptr);
     });
 }

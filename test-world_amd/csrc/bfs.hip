@@ -28,9 +28,22 @@ struct BfsTopDown {
 
     __device__ __forceinline__ uint32_t vertex(uint32_t) const { return 0; }
 
-    __device__ __forceinline__ void edge(bool valid, uint32_t, uint64_t e, const Emit &em, Acc &acc) const {
-        const uint32_t w = valid ? col[e] : 0u;
-        const bool fresh = valid && level[w] == kUnreached;
+    struct Pre {
+        uint32_t w, lv;
+        bool valid;
+    };
+
+    __device__ __forceinline__ Pre load(bool valid, uint32_t, uint64_t e) const {
+        Pre p;
+        p.valid = valid;
+        p.w = valid ? col[e] : 0u;
+        p.lv = valid ? level[p.w] : 0u;
+        return p;
+    }
+
+    __device__ __forceinline__ void apply(const Pre &p, const Emit &em, Acc &acc) const {
+        const uint32_t w = p.w;
+        const bool fresh = p.valid && p.lv == kUnreached;
         if (em.mode == kModeQueue) {
             const uint32_t bit = 1u << (w & 31);
             bool won = false;

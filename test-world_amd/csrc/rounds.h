@@ -173,6 +173,14 @@ __device__ __forceinline__ void add_tallies(RoundCtr *out, const Acc &acc) {
     }
 }
 
+// Edges are taken kGroup at a time: every edge's loads (column, weight, the
+// target's value) are issued before any is applied (compare, atomic, emit),
+// so a lane keeps kGroup random reads in flight. A value read before another
+// edge of the group is applied can only be larger than the current one
+// (values only fall), so it lets at most more candidates reach the atomic,
+// which decides.
+constexpr uint32_t kGroup = 4;
+
 // Lane-per-vertex expansion of the tile's vertices of degree 1..sd (`mem`
 // says whether the lane holds a frontier vertex). Wave-uniform call.
 template <class Op>
@@ -187,7 +195,14 @@ __device__ __forceinline__ void process_tile(const RoundEnv &e, bool mem, uint32
     const uint32_t ds = s ? d : 0u;
     const uint32_t sv = s ? op.vertex(v) : 0u;
     const uint32_t md = wave_max_u32(ds);
-    for (uint32_t k = 0; k < md; k++) op.edge(k < ds, sv, (uint64_t)b + k, em, acc);
+    for (uint32_t k = 0; k < md; k += kGroup) {
+        typename Op::Pre pre[kGroup];
+#pragma unroll
+        for (uint32_t j = 0; j < kGroup; j++) pre[j] = op.load(k + j < ds, sv, (uint64_t)b + k + j);
+#pragma unroll
+        for (uint32_t j = 0; j < kGroup; j++)
+            if (k + j < md) op.apply(pre[j], em, acc);
+    }
 }
 
 // Walks the chunk queue: wave gw takes the contiguous global chunk range
@@ -238,7 +253,16 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
         }
         const uint32_t b = k * kWaveChunk;
         const uint32_t end = min(vd, b + kWaveChunk);
-        for (uint32_t t = b; t < end; t += 64) op.edge(t + lane < end, sv, (uint64_t)vb + t + lane, em, acc);
+        static_assert(kWaveChunk == 64 * kGroup, "a chunk is one group of 64-edge steps");
+        typename Op::Pre pre[kGroup];
+#pragma unroll
+        for (uint32_t j = 0; j < kGroup; j++) {
+            const uint32_t t = b + 64 * j + lane;
+            pre[j] = op.load(t < end, sv, (uint64_t)vb + t);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kGroup; j++)
+            if (b + 64 * j < end) op.apply(pre[j], em, acc);
         k++;
     }
 }

@@ -43,11 +43,25 @@ struct Relax {
 
     __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return val[v]; }
 
-    __device__ __forceinline__ void edge(bool valid, uint32_t sv, uint64_t e, const Emit &em, Acc &acc) const {
-        const uint32_t w = valid ? col[e] : 0u;
-        const uint32_t nv = kWeighted ? sv + (valid ? wt[e] : 0u) : sv;
+    struct Pre {
+        uint32_t w, nv, cur;
+        bool valid;
+    };
+
+    // Loads only: the edge's target, its candidate value and the target's value.
+    __device__ __forceinline__ Pre load(bool valid, uint32_t sv, uint64_t e) const {
+        Pre p;
+        p.valid = valid;
+        p.w = valid ? col[e] : 0u;
+        p.nv = kWeighted ? sv + (valid ? wt[e] : 0u) : sv;
+        p.cur = valid ? val[p.w] : 0u;
+        return p;
+    }
+
+    __device__ __forceinline__ void apply(const Pre &p, const Emit &em, Acc &acc) const {
+        const uint32_t w = p.w, nv = p.nv;
         bool improved = false;
-        if (valid && nv < val[w]) improved = nv < atomicMin(&val[w], nv);
+        if (p.valid && nv < p.cur) improved = nv < atomicMin(&val[w], nv);
         const bool near = improved && nv < thr;
         if (improved && !near) {
             far[w] = 1;
@@ -118,11 +132,20 @@ struct HeavyRelax {
     uint32_t lo, hi;
     __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return val[v]; }
     __device__ __forceinline__ bool active(uint32_t d) const { return d >= lo && d < hi; }
-    __device__ __forceinline__ void edge(bool valid, uint32_t d, uint64_t e) const {
-        if (!valid) return;
-        const uint32_t w = col[e];
-        const uint32_t nv = d + wt[e];
-        if (nv < val[w] && nv < atomicMin(&val[w], nv)) far[w] = 1;
+    struct Pre {
+        uint32_t w, nv, cur;
+        bool valid;
+    };
+    __device__ __forceinline__ Pre load(bool valid, uint32_t d, uint64_t e) const {
+        Pre p;
+        p.valid = valid;
+        p.w = valid ? col[e] : 0u;
+        p.nv = valid ? d + wt[e] : 0u;
+        p.cur = valid ? val[p.w] : 0u;
+        return p;
+    }
+    __device__ __forceinline__ void apply(const Pre &p) const {
+        if (p.valid && p.nv < p.cur && p.nv < atomicMin(&val[p.w], p.nv)) far[p.w] = 1;
     }
 };
 

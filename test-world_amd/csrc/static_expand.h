@@ -2,9 +2,22 @@
 // the graph is created): CC hooking and the symmetry checks use it.
 #pragma once
 
+#include <type_traits>
+
 #include "device_util.h"
 
 namespace ptg {
+
+// Ops with a Pre type split an edge into load() (reads only) and apply();
+// the walks below then issue kStaticGroup edges' loads before applying any
+// (a value read before another edge of the group is applied can only be
+// larger than the current one, so at most more candidates reach the op's
+// atomic, which decides).
+constexpr uint32_t kStaticGroup = 4;
+template <class T, class = void>
+struct HasPre : std::false_type {};
+template <class T>
+struct HasPre<T, std::void_t<typename T::Pre>> : std::true_type {};
 
 // op.active(sv) when the op defines it (vertices it returns false for are
 // skipped), else true.
@@ -38,7 +51,20 @@ __device__ __forceinline__ void expand_static(const Frontier &in, const uint32_t
         const uint64_t e = min((uint64_t)off[v + 1], b + kChunkEdges);
         const auto sv = op.vertex(v);
         if (!op_active(op, sv, 0)) continue;
-        for (uint64_t base = b + wib * 64; base < e; base += kBlock) op.edge(base + lane < e, sv, base + lane);
+        if constexpr (HasPre<Op>::value) {
+            for (uint64_t base = b + wib * 64; base < e; base += (uint64_t)kBlock * kStaticGroup) {
+                typename Op::Pre pre[kStaticGroup];
+#pragma unroll
+                for (uint32_t j = 0; j < kStaticGroup; j++) {
+                    const uint64_t i = base + (uint64_t)j * kBlock + lane;
+                    pre[j] = op.load(i < e, sv, i);
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kStaticGroup; j++) op.apply(pre[j]);
+            }
+        } else {
+            for (uint64_t base = b + wib * 64; base < e; base += kBlock) op.edge(base + lane < e, sv, base + lane);
+        }
     }
     const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
@@ -48,7 +74,20 @@ __device__ __forceinline__ void expand_static(const Frontier &in, const uint32_t
         const auto sv = op.vertex(v);
         if (!op_active(op, sv, 0)) continue;
         const uint64_t b = off[v], e = off[v + 1];
-        for (uint64_t base = b; base < e; base += 64) op.edge(base + lane < e, sv, base + lane);
+        if constexpr (HasPre<Op>::value) {
+            for (uint64_t base = b; base < e; base += 64 * kStaticGroup) {
+                typename Op::Pre pre[kStaticGroup];
+#pragma unroll
+                for (uint32_t j = 0; j < kStaticGroup; j++) {
+                    const uint64_t i = base + 64 * j + lane;
+                    pre[j] = op.load(i < e, sv, i);
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kStaticGroup; j++) op.apply(pre[j]);
+            }
+        } else {
+            for (uint64_t base = b; base < e; base += 64) op.edge(base + lane < e, sv, base + lane);
+        }
     }
     const uint32_t ns = in.ctr->n_s;
     for (uint64_t base = (uint64_t)gw * 64; base < ns; base += (uint64_t)nw * 64) {
@@ -65,7 +104,17 @@ __device__ __forceinline__ void expand_static(const Frontier &in, const uint32_t
             }
         }
         const uint32_t md = wave_max_u32(d);
-        for (uint32_t j = 0; j < md; j++) op.edge(j < d, sv, (uint64_t)b + j);
+        if constexpr (HasPre<Op>::value) {
+            for (uint32_t j = 0; j < md; j += kStaticGroup) {
+                typename Op::Pre pre[kStaticGroup];
+#pragma unroll
+                for (uint32_t t = 0; t < kStaticGroup; t++) pre[t] = op.load(j + t < d, sv, (uint64_t)b + j + t);
+#pragma unroll
+                for (uint32_t t = 0; t < kStaticGroup; t++) op.apply(pre[t]);
+            }
+        } else {
+            for (uint32_t j = 0; j < md; j++) op.edge(j < d, sv, (uint64_t)b + j);
+        }
     }
 }
 

@@ -456,6 +456,7 @@ def main():
         },
         "bfs_gteps": round(bfs_gteps, 4),
         "sssp_gteps": round(sssp_gteps, 4),
+        "rounds_per_query": {k: round(tot(k, 5) / max(1, len(acc[k])), 2) for k in ("bfs", "sssp")},
         "pagerank_iters_per_s": round(pr_iters_s, 2),
         "cc_ms": round(cc_ms, 3),
         # every kind's main kernel, timed in the warmup steps

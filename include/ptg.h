@@ -168,6 +168,11 @@ PTG_API void ptg_graph_destroy(ptg_graph *g);
 PTG_API uint64_t ptg_graph_estimate_bytes(uint32_t V, uint64_t E, int weighted, const ptg_config *cfg);
 PTG_API uint64_t ptg_graph_device_bytes(const ptg_graph *g);
 PTG_API int ptg_graph_is_symmetric(ptg_graph *g); /* 1, 0, or negative error */
+/* Frontier size of every round of the last ptg_bfs / ptg_sssp / ptg_cc (label
+ * propagation) on this graph, the per-iteration counterpart of the reference's
+ * `nodeSum` (ptGraph/main.cu:1109-1113): copies up to `cap` into `sizes`,
+ * sets *count to the number of rounds. */
+PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32_t cap, uint32_t *count);
 /* Changes the graph's runtime flags. Only PTG_FLAG_TIMING may differ from the
  * flags the graph was created with (PTG_ERR_INVALID_ARG otherwise). No
  * reference counterpart: the reference times whole calls with the host clock. */

@@ -179,6 +179,9 @@ struct ptg_graph {
     // PTG_FLAG_STREAM: host-resident edges and the device chunk cache (stream.hip).
     ptg::StreamState *streamed = nullptr;
 
+    // Frontier size of every round of the last BFS / SSSP / label-propagation query.
+    std::vector<uint64_t> frontiers;
+
     int symmetric = -1;  // -1 unknown, 0/1 once checked
     uint64_t device_bytes = 0;                          // charged against the budget
     uint64_t create_bytes = 0;                          // device_bytes when creation finished

@@ -344,6 +344,7 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t *rounds_out, u
     uint32_t r = 0, batch = 2, rounds = 0;
     uint64_t fs = 0, es = 0;
     uint32_t checked = 0;  // ring entries [0, checked) already summed
+    g->frontiers.clear();
     for (;;) {
         for (uint32_t k = 0; k < batch; k++, r++) {
             int rc = launch(r);
@@ -362,6 +363,7 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t *rounds_out, u
             const HostSummary s = host_summary(c);
             fs += s.found;
             es += s.edges;
+            g->frontiers.push_back(s.found);
             rounds++;
         }
         if (done) break;

@@ -577,6 +577,14 @@ int ptg_graph_get_config(const ptg_graph *g, ptg_config *c) {
     return PTG_OK;
 }
 
+int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32_t cap, uint32_t *count) {
+    if (!g || !count || (cap && !sizes)) return fail(PTG_ERR_INVALID_ARG, "null argument");
+    const uint32_t n = (uint32_t)g->frontiers.size();
+    for (uint32_t i = 0; i < n && i < cap; i++) sizes[i] = g->frontiers[i];
+    *count = n;
+    return PTG_OK;
+}
+
 int ptg_graph_set_flags(ptg_graph *g, uint32_t flags) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if ((flags ^ g->cfg.flags) & ~PTG_FLAG_TIMING)

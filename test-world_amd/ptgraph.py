@@ -193,6 +193,8 @@ SIGNATURES = {
     "ptg_graph_device_bytes": (ctypes.c_uint64, [_vp]),
     "ptg_graph_is_symmetric": (ctypes.c_int, [_vp]),
     "ptg_graph_set_flags": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "ptg_graph_frontier_sizes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+                                                ctypes.POINTER(ctypes.c_uint32)]),
     "ptg_graph_set_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
     "ptg_graph_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
     "ptg_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
@@ -520,6 +522,14 @@ class Graph:
         if rc < 0:
             raise PtgError(rc, "ptg_graph_is_symmetric")
         return bool(rc)
+
+    def frontier_sizes(self):
+        """Frontier size of every round of the last BFS / SSSP / CC query."""
+        n = ctypes.c_uint32(0)
+        _check(lib().ptg_graph_frontier_sizes(self._h, None, 0, ctypes.byref(n)), "ptg_graph_frontier_sizes")
+        buf = (ctypes.c_uint64 * max(1, n.value))()
+        _check(lib().ptg_graph_frontier_sizes(self._h, buf, n.value, ctypes.byref(n)), "ptg_graph_frontier_sizes")
+        return np.array(buf[: n.value], dtype=np.uint64)
 
     def set_flags(self, flags):
         """Changes runtime flags (only FLAG_TIMING may change after creation)."""

@@ -310,3 +310,15 @@ def test_sssp_steps_rmat16(rmat16):
                 np.testing.assert_array_equal(d, want[s], err_msg="delta %d src %d" % (delta, s))
     finally:
         gg.tune(sssp_delta=0)
+
+
+def test_frontier_sizes_match_levels(rmat16):
+    g, gg = rmat16
+    say("case frontier sizes")
+    for s in (0, 100):
+        lv, st = gg.bfs(s)
+        fr = gg.frontier_sizes()
+        hist = np.bincount(lv[lv != 0xFFFFFFFF])[1:]  # vertices per level 1, 2, ...
+        assert fr.size == st.iterations == hist.size
+        np.testing.assert_array_equal(fr, hist)
+        assert int(fr.sum()) == st.frontier_sum == st.reached

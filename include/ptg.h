@@ -80,7 +80,8 @@ typedef struct ptg_config {
     uint32_t large_degree;        /* degree >  large_degree: split over workgroups */
     float td_to_bu;               /* BFS switches to bottom-up when m_f > m_u / td_to_bu */
     float bu_to_td;               /* BFS returns to top-down when n_f < V / bu_to_td */
-    uint32_t sssp_delta;          /* near/far split step for SSSP; 0 = auto */
+    uint32_t sssp_delta;          /* near/far bucket step for SSSP (edges of weight < step are
+                                     light); 0 = one unbounded bucket, no split */
     uint32_t flags;               /* PTG_FLAG_* */
     uint32_t batch_delta;         /* near/far step of batched SSSP; 0 = auto */
     uint32_t batch_grid;          /* workgroups of the batched kernels; 0 = grid_blocks */
@@ -88,6 +89,10 @@ typedef struct ptg_config {
     uint32_t batch_chunk;         /* edges per static chunk of batched SSSP; 0 = auto (256) */
     uint32_t order_hub_degree;    /* vertex order on the device: 0 = by descending degree; d > 0 =
                                      vertices of degree > d by degree, then breadth-first from them */
+    uint32_t sssp_defer_degree;   /* SSSP: a vertex of degree > this waits in the frontier while its
+                                     distance >= the frontier minimum + sssp_defer_window;
+                                     0 = auto (64), 0xFFFFFFFF = never */
+    uint32_t sssp_defer_window;   /* 0 = auto (mean weight / 16, at least 1) */
     uint32_t reserved;
 } ptg_config;
 
@@ -177,8 +182,9 @@ PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32
  * flags the graph was created with (PTG_ERR_INVALID_ARG otherwise). No
  * reference counterpart: the reference times whole calls with the host clock. */
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
-/* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta, batch_delta,
- * batch_grid, batch_td_to_bu, batch_chunk and PTG_FLAG_TIMING. Every other field must equal
+/* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta,
+ * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid,
+ * batch_td_to_bu, batch_chunk and PTG_FLAG_TIMING. Every other field must equal
  * the graph's (PTG_ERR_INVALID_ARG otherwise): they shape allocations made at
  * creation. Results never depend on these values, only speed does. */
 PTG_API int ptg_graph_set_config(ptg_graph *g, const ptg_config *cfg);
@@ -217,7 +223,7 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
  * on the tuning, only speed does. No reference counterpart (the reference
  * hard-codes 56 x 1024 threads and its partition fraction, main.cu:318-319). */
 #define PTG_TUNE_BFS 0x1u         /* td_to_bu, bu_to_td */
-#define PTG_TUNE_SSSP 0x2u        /* sssp_delta */
+#define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
 #define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_grid, batch_chunk */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);

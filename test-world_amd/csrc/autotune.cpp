@@ -1,5 +1,5 @@
 // Autotuner: picks the graph's runtime tuning (BFS push/pull switch
-// thresholds, SSSP near/far steps, launch width and static chunk size of the
+// thresholds, SSSP bucket step and deferral, launch width and static chunk size of the
 // batched kernels) by timing the graph's own queries.
 //
 // Coordinate search: each knob in turn is set to every value of its grid with
@@ -96,7 +96,6 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     // explicit values for every knob the search moves
     if (c.td_to_bu <= 0) c.td_to_bu = 14.0f;
     if (c.bu_to_td <= 0) c.bu_to_td = 24.0f;
-    if (!c.sssp_delta) c.sssp_delta = 24;
     if (c.batch_td_to_bu <= 0) c.batch_td_to_bu = c.td_to_bu;
     if (!c.batch_grid) c.batch_grid = 2 * g->grid;
     if (!c.batch_delta) c.batch_delta = 256;
@@ -110,8 +109,13 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kBfs, &c.td_to_bu, std::vector<float>{4, 8, 14, 24, 40});
         if (!rc) rc = t.knob(kBfs, &c.bu_to_td, std::vector<float>{8, 16, 24, 48});
     }
-    if ((what & PTG_TUNE_SSSP) && g->weighted && !rc)
-        rc = t.knob(kSssp, &c.sssp_delta, std::vector<uint32_t>{12, 16, 24, 32, 48});
+    if ((what & PTG_TUNE_SSSP) && g->weighted) {
+        // 0 = automatic (one unbounded bucket; degree 64; mean weight / 16)
+        if (!rc) rc = t.knob(kSssp, &c.sssp_delta, std::vector<uint32_t>{0, 24, 64, 128});
+        if (!rc) rc = t.knob(kSssp, &c.sssp_defer_window, std::vector<uint32_t>{0, 1, 3, 4, 6});
+        if (!rc) rc = t.knob(kSssp, &c.sssp_defer_degree, std::vector<uint32_t>{0, 32, 128, 256, 0xFFFFFFFFu});
+        if (!rc && !c.sssp_delta) release_sssp_split(g);  // the light/heavy CSRs tried above
+    }
     if (what & PTG_TUNE_BFS_BATCH) {
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_td_to_bu, std::vector<float>{2, 6, 14, 30});
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G});

@@ -52,7 +52,9 @@ struct RoundCtr {
     uint32_t far_any;                       // SSSP: far set non-empty after the producing round
     uint32_t need_heavy;                    // SSSP: the bucket's heavy edges are still to relax
     uint32_t lo;                            // SSSP: lower end of the current bucket
-    uint32_t pad[7];
+    uint32_t fmin_inv;                      // SSSP: ~(smallest value in this frontier); 0 = unknown
+    uint32_t ndefer;                        // SSSP: entries of the deferred list (pre-pass)
+    uint32_t pad[5];
     uint32_t found_sh[kShards];             // frontier vertices (any degree)
     uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
     unsigned long long edges_sh[kShards];   // sum of out-degrees of the frontier
@@ -173,6 +175,10 @@ struct ptg_graph {
     uint32_t split_delta = 0;
     ptg::Csr light, heavy;
     uint32_t *d_wl = nullptr, *d_wh = nullptr;
+    // SSSP deferred high-degree vertices of a round (V entries) and the
+    // automatic deferral window (0 = not computed yet).
+    uint32_t *d_defer = nullptr;
+    uint32_t auto_window = 0;
 
     // Batched multi-source state (batch.hip), allocated on the first batch.
     ptg::BatchState *batch = nullptr;

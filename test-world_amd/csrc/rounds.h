@@ -23,6 +23,15 @@
 // per lane from the S queue or straight from bitmap tiles of 64 vertices.
 // Counters are summed per workgroup and added to 1 of 16 shards (a single hot
 // address serialises atomics: measured 11 ns each on MI355X).
+//
+// SSSP defers high-degree vertices: one of degree > defer_deg whose value is
+// at or above the frontier's smallest value plus `window` stays in the
+// frontier without being expanded, so that a hub is expanded close to its
+// final value rather than once per improvement. The pre-pass moves such
+// vertices of flags and bitmap frontiers to a deferred list, which the round
+// puts back into its output frontier; queue frontiers (at most kQueueCap
+// edges) skip them during the walk. The vertex with the smallest value is
+// always expanded, so every round makes progress.
 #pragma once
 
 #include "device_util.h"
@@ -44,13 +53,28 @@ struct RoundEnv {
     uint32_t V, npairs, nwords;
     uint32_t sd;                  // small degree threshold
     uint32_t qcap;                // frontier edges up to which the output is queued
+    const uint32_t *val;          // SSSP deferral: vertex values
+    uint32_t *dq;                 // SSSP deferral: deferred list of the round
+    uint32_t defer_deg;           // defer vertices of degree > defer_deg (~0u: never)
+    uint32_t window;              // ... whose value >= frontier minimum + window
 };
+
+// Saturating a + b.
+__device__ __forceinline__ uint32_t add_sat(uint32_t a, uint32_t b) { return a > 0xFFFFFFFFu - b ? 0xFFFFFFFFu : a + b; }
+
+// Values at or above this limit defer a vertex of degree > defer_deg in the
+// round that reads `in` (no limit when the frontier minimum is unknown).
+__device__ __forceinline__ uint32_t defer_limit(const RoundCtr *in, uint32_t window) {
+    const uint32_t fi = in->fmin_inv;
+    return fi ? add_sat(~fi, window) : 0xFFFFFFFFu;
+}
 
 // Per-thread tallies, summed per workgroup at the end of a round.
 struct Acc {
     uint32_t found = 0;
     uint32_t far = 0;
     uint64_t edges = 0;
+    uint32_t fmin_inv = 0;  // max of ~value over the vertices put into the output frontier
 };
 
 // Output side of a round.
@@ -146,32 +170,47 @@ __device__ __forceinline__ void clear_bitmap_ahead(const RoundEnv &e) {
 // Adds the workgroup's tallies to shard blockIdx % kShards of `out`.
 // Every thread of the workgroup must call it.
 __device__ __forceinline__ void add_tallies(RoundCtr *out, const Acc &acc) {
-    __shared__ uint32_t s_found[kWavesPerBlock], s_far[kWavesPerBlock];
+    __shared__ uint32_t s_found[kWavesPerBlock], s_far[kWavesPerBlock], s_fmin[kWavesPerBlock];
     __shared__ unsigned long long s_edges[kWavesPerBlock];
     const uint32_t f = (uint32_t)wave_sum_u64(acc.found);
     const uint32_t fa = (uint32_t)wave_sum_u64(acc.far);
     const uint64_t ed = wave_sum_u64(acc.edges);
+    const uint32_t fm = wave_max_u32(acc.fmin_inv);
     const uint32_t wib = threadIdx.x >> 6;
     if (lane_id() == 0) {
         s_found[wib] = f;
         s_far[wib] = fa;
         s_edges[wib] = ed;
+        s_fmin[wib] = fm;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t tf = 0, tfa = 0;
+        uint32_t tf = 0, tfa = 0, tm = 0;
         unsigned long long te = 0;
         for (int w = 0; w < kWavesPerBlock; w++) {
             tf += s_found[w];
             tfa += s_far[w];
             te += s_edges[w];
+            tm = max(tm, s_fmin[w]);
         }
         const uint32_t shard = blockIdx.x % kShards;
         if (tf) atomicAdd(&out->found_sh[shard], tf);
         if (tfa) atomicAdd(&out->far_sh[shard], tfa);
         if (te) atomicAdd(&out->edges_sh[shard], te);
+        if (tm > out->fmin_inv) atomicMax(&out->fmin_inv, tm);
     }
 }
+
+// Ops that defer high-degree vertices (SSSP relaxation) provide
+// defers(degree, value) and requeue(pred, v, degree, value, emit, acc).
+template <class Op, class = void>
+struct HasDefer {
+    static constexpr bool value = false;
+};
+template <class Op>
+struct HasDefer<Op, decltype((void)&Op::requeue)> {
+    static constexpr bool value = true;
+};
 
 // Edges are taken kGroup at a time: every edge's loads (column, weight, the
 // target's value) are issued before any is applied (compare, atomic, emit),
@@ -235,6 +274,8 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
     uint32_t vb = e.off[v], vd = e.off[v + 1] - vb;
     uint32_t k = local - bases[idx];
     uint32_t sv = op.vertex(v);
+    bool skip = false;
+    if constexpr (HasDefer<Op>::value) skip = op.defers(vd, __builtin_amdgcn_readfirstlane(sv));
     for (uint32_t c = c0; c < c1; c++) {
         while (k >= wave_chunks(vd)) {  // next entry, possibly in the next shard
             k = 0;
@@ -250,6 +291,20 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
             vb = e.off[v];
             vd = e.off[v + 1] - vb;
             sv = op.vertex(v);
+            if constexpr (HasDefer<Op>::value) skip = op.defers(vd, __builtin_amdgcn_readfirstlane(sv));
+        }
+        if constexpr (HasDefer<Op>::value) {
+            // A deferred entry (only queue frontiers reach here with one: the
+            // pre-pass filters the others) goes back into the output frontier
+            // once, from the wave that holds its first chunk. A wave that read
+            // an older, larger value and skipped a later chunk of an entry that
+            // another wave expanded left it to the improvement in between,
+            // which put the vertex into the output frontier.
+            if (skip) {
+                if (k == 0) op.requeue(lane == 0, v, vd, __builtin_amdgcn_readfirstlane(sv), em, acc);
+                k++;
+                continue;
+            }
         }
         const uint32_t b = k * kWaveChunk;
         const uint32_t end = min(vd, b + kWaveChunk);
@@ -300,6 +355,23 @@ __device__ __forceinline__ void expand_frontier(const RoundEnv &e, const RoundCt
             if (ballot(mem) == 0) continue;
             process_tile(e, mem, p * 64 + lane, op, em, acc);
         }
+    }
+}
+
+// Puts the pre-pass's deferred vertices of round e.r back into the output
+// frontier, a lane per vertex.
+template <class Op>
+__device__ __forceinline__ void requeue_deferred(const RoundEnv &e, const RoundCtr *in, Op &op, const Emit &em,
+                                                 Acc &acc) {
+    const uint32_t n = in->ndefer;
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    for (uint32_t t = gw * 64; t < n; t += nw * 64) {
+        const uint32_t i = t + lane_id();
+        const bool valid = i < n;
+        const uint32_t v = valid ? e.dq[i] : 0u;
+        const uint32_t d = valid ? e.off[v + 1] - e.off[v] : 0u;
+        op.requeue(valid, v, d, valid ? op.vertex(v) : 0u, em, acc);
     }
 }
 

@@ -32,6 +32,10 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
     e.nwords = g->nwords;
     e.sd = g->cfg.small_degree;
     e.qcap = kQueueCap;
+    e.val = nullptr;
+    e.dq = g->d_defer;
+    e.defer_deg = 0xFFFFFFFFu;
+    e.window = 0;
     return e;
 }
 
@@ -39,7 +43,9 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
 // tiles [b*T, (b+1)*T): for a flags frontier it packs (and clears) the bytes
 // into bitmap words and counts the frontier; for both it gathers members of
 // degree > sd in LDS and appends them to chunk-queue shard b % 16 with one
-// atomic per batch (an LDS scan gives the chunk bases).
+// atomic per batch (an LDS scan gives the chunk bases). Members that the
+// round defers (degree > defer_deg, value >= the frontier minimum + window)
+// go to the deferred list instead.
 __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
     constexpr uint32_t kBuf = 1024;
     __shared__ uint32_t s_v[kBuf], s_n[kBuf];
@@ -54,6 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
     const uint32_t t_begin = blockIdx.x * T;
     const uint32_t t_end = min(e.npairs, t_begin + T);
     const uint32_t shard = blockIdx.x % kShards;
+    const uint32_t lim = e.defer_deg != 0xFFFFFFFFu ? defer_limit(in, e.window) : 0xFFFFFFFFu;
     uint32_t *ids = e.cqin + (uint64_t)shard * e.cq_cap;
     uint32_t *bases = e.cbin + (uint64_t)shard * e.cq_cap;
     Acc acc;
@@ -79,7 +86,12 @@ __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
             acc.found++;
             acc.edges += d;
         }
-        const bool want = mem && d > e.sd;
+        bool want = mem && d > e.sd;
+        if (lim != 0xFFFFFFFFu) {
+            const bool defer = want && d > e.defer_deg && e.val[v] >= lim;
+            want = want && !defer;
+            wave_append(e.dq, &in->ndefer, defer, v);
+        }
         const uint64_t m = ballot(want);
         if (m) {
             uint32_t base = 0;
@@ -142,6 +154,7 @@ __global__ void k_seed(RoundEnv e, uint32_t src, uint32_t src_value, uint32_t *v
     c->dir = kDirTopDown;
     c->thr = thr;
     c->lo = src_value;
+    c->fmin_inv = ~src_value;
     c->any = 1u;
     c->found_sh[0] = 1;
     c->edges_sh[0] = d;

@@ -13,6 +13,14 @@
 // (weight < delta); when the bucket's frontier is empty its vertices' values
 // are final, and one round relaxes their heavy edges once, every one of which
 // lands at or above the bucket's end and so in the far set.
+//
+// By default there is a single unbounded bucket (every edge light, no far
+// set): small vertices are relaxed as soon as they improve, and vertices of
+// degree > defer_deg wait in the frontier until their value is within a
+// window of the frontier's smallest value (rounds.h). On R-MAT graphs this
+// scans each hub's list about once instead of once per improvement. Only the
+// order of relaxations changes, so the values are the same.
+#include <algorithm>
 #include <vector>
 
 #include <string.h>
@@ -22,9 +30,10 @@
 
 namespace ptg {
 
-// Default near/far step: measured best on R-MAT graphs with weights in
-// [1, 64] (24 against 16: 3.5 against 3.8 ms per query at scale 22).
-constexpr uint32_t kDefaultDelta = 24;
+// sssp_delta 0: one unbounded bucket, no light/heavy split.
+constexpr uint32_t kUnbounded = 0xFFFFFFFFu;
+// Default deferral degree: vertices of degree > 64 wait for the window.
+constexpr uint32_t kDefaultDeferDegree = 64;
 
 // One relaxation. Queue mode dedupes near vertices through the output
 // bitmap so each is appended once; flags mode (large frontiers) stores a byte
@@ -40,8 +49,29 @@ struct Relax {
     uint8_t *flout;
     uint8_t *far;
     uint32_t thr;
+    uint32_t defer_deg, lim;  // vertices of degree > defer_deg and value >= lim wait
 
     __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return val[v]; }
+
+    __device__ __forceinline__ bool defers(uint32_t d, uint32_t sv) const { return d > defer_deg && sv >= lim; }
+
+    // Puts a deferred vertex (value sv, degree d) back into the output
+    // frontier for lanes with pred. Wave-uniform.
+    __device__ __forceinline__ void requeue(bool pred, uint32_t v, uint32_t d, uint32_t sv, const Emit &em,
+                                            Acc &acc) const {
+        if (pred) acc.fmin_inv = max(acc.fmin_inv, ~sv);
+        if (em.mode == kModeQueue) {
+            bool won = false;
+            if (pred) {
+                const uint32_t bit = 1u << (v & 31);
+                won = !(atomicOr(&fout[v >> 5], bit) & bit);
+            }
+            emit(em, won, v, d, acc);
+        } else {
+            if (pred) flout[v] = 1;
+            mark_any(em, pred);
+        }
+    }
 
     struct Pre {
         uint32_t w, nv, cur;
@@ -63,6 +93,7 @@ struct Relax {
         bool improved = false;
         if (p.valid && nv < p.cur) improved = nv < atomicMin(&val[w], nv);
         const bool near = improved && nv < thr;
+        if (near) acc.fmin_inv = max(acc.fmin_inv, ~nv);
         if (improved && !near) {
             far[w] = 1;
             acc.far = 1;
@@ -107,6 +138,7 @@ __device__ __forceinline__ void sssp_split(const RoundEnv &e, const uint32_t *va
             if (bits) e.fout[p * 2 + (lane >> 5)] = bits;
         }
         if (keep) acc.far = 1;
+        if (move) acc.fmin_inv = max(acc.fmin_inv, ~dv);
         emit(em, move, v, d, acc);
     }
 }
@@ -119,6 +151,8 @@ struct RelaxParams {
     uint32_t split;      // 1: heavy edges live in the CSR below and get a round per bucket
     const uint32_t *hoff, *hcol, *hw;
     Frontier hbins;      // static degree bins of the heavy CSR
+    uint32_t defer_deg;  // ~0u: no deferral
+    uint32_t window;
 };
 
 // Heavy-edge round: every vertex whose final value lies in the bucket
@@ -183,8 +217,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_round(RoundEnv e, RelaxParams 
     const Emit em = make_emit(e, out, mode);
     Acc acc;
     if (expand) {
-        Relax<kWeighted> op{e.off, e.col, p.wt, p.val, e.fout, e.flout, p.far, thr};
+        const uint32_t lim = p.defer_deg != 0xFFFFFFFFu ? defer_limit(in, p.window) : 0xFFFFFFFFu;
+        Relax<kWeighted> op{e.off, e.col, p.wt, p.val, e.fout, e.flout, p.far, thr, p.defer_deg, lim};
         expand_frontier(e, in, op, em, acc);
+        if (lim != 0xFFFFFFFFu) requeue_deferred(e, in, op, em, acc);
     } else if (heavy) {
         expand_static(p.hbins, p.hoff, HeavyRelax{p.hcol, p.hw, p.val, p.far, lo, thr});
     } else {
@@ -201,7 +237,10 @@ static int run_relax_rounds(ptg_graph *g, RelaxParams p, const uint32_t *off, co
     return drive_rounds(
         g,
         [&](uint32_t r) -> int {
-            const RoundEnv env = make_env_csr(g, r, off, col);
+            RoundEnv env = make_env_csr(g, r, off, col);
+            env.val = p.val;
+            env.defer_deg = p.defer_deg;
+            env.window = p.window;
             int rc = launch_prepare(g, env);
             if (rc) return rc;
             timer.launch(k_relax_round<kWeighted>, dim3(g->grid), dim3(kBlock), env, p);
@@ -317,31 +356,76 @@ static int ensure_split(ptg_graph *g, uint32_t delta) {
     return PTG_OK;
 }
 
-void release_sssp_split(ptg_graph *g) { free_split(g); }
+void release_sssp_split(ptg_graph *g) {
+    free_split(g);
+    gfree(g, g->d_defer);
+    g->d_defer = nullptr;
+}
 
 uint64_t sssp_split_need(const ptg_graph *g) {
-    const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kDefaultDelta;
-    if (!g->weighted || (g->split_delta == delta && g->light.off)) return 0;
-    return 8ull * g->E + 8ull * (g->V + 1) + 4ull * g->V + (64u << 10);
+    if (!g->weighted) return 0;
+    uint64_t need = g->d_defer ? 0 : 4ull * g->V;
+    const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kUnbounded;
+    if (delta != kUnbounded && !(g->split_delta == delta && g->light.off))
+        need += 8ull * g->E + 8ull * (g->V + 1) + 4ull * g->V + (64u << 10);
+    return need;
+}
+
+// Sum of the weights, over 16 accumulators.
+__global__ __launch_bounds__(kBlock) void k_weight_sum(const uint32_t *__restrict__ w, uint64_t E,
+                                                       unsigned long long *acc) {
+    uint64_t sum = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < E; i += (uint64_t)gridDim.x * kBlock) sum += w[i];
+    sum = wave_sum_u64(sum);
+    if (lane_id() == 0) atomicAdd(&acc[(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kShards], sum);
+}
+
+// Allocates the deferred list and, once per graph, sets the automatic
+// deferral window: the mean weight / 16, at least 1 (2 for weights uniform
+// in [1, 64]).
+static int ensure_defer(ptg_graph *g) {
+    int rc;
+    if (!g->d_defer && (rc = galloc(g, (void **)&g->d_defer, 4ull * std::max<uint32_t>(g->V, 1)))) return rc;
+    if (g->auto_window) return PTG_OK;
+    unsigned long long sums[kShards];
+    if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_acc), 2 * kShards, 0u))) return rc;
+    if (g->E) {
+        k_weight_sum<<<g->grid, kBlock, 0, g->stream>>>(g->d_w, g->E, g->d_acc);
+        PTG_HIP(hipGetLastError());
+    }
+    if ((rc = copy_sync(g, sums, g->d_acc, sizeof sums, hipMemcpyDeviceToHost))) return rc;
+    uint64_t total = 0;
+    for (int i = 0; i < kShards; i++) total += sums[i];
+    const uint64_t mean = g->E ? total / g->E : 0;
+    g->auto_window = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((mean + 8) / 16, 1u << 30));
+    return PTG_OK;
 }
 
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     int rc;
     LaunchTimer timer(g);
-    const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kDefaultDelta;
-    if ((rc = ensure_split(g, delta))) return rc;
+    const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kUnbounded;
+    const bool split = delta != kUnbounded;
+    if (split && (rc = ensure_split(g, delta))) return rc;
+    if ((rc = ensure_defer(g))) return rc;
+    const uint32_t defer_deg = g->cfg.sssp_defer_degree ? g->cfg.sssp_defer_degree : kDefaultDeferDegree;
+    const uint32_t window = g->cfg.sssp_defer_window ? g->cfg.sssp_defer_window : g->auto_window;
+    const uint32_t *off = split ? g->light.off : g->out.off;
+    const uint32_t *col = split ? g->light.col : g->out.col;
     PTG_HIP(hipEventRecord(g->ev0, g->stream));
     if ((rc = fill_u32(g, dist, g->V, g->cfg.sssp_sentinel)) ||
-        (rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_far), (uint64_t)g->npairs * 16, 0u)) ||
+        (split && (rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_far), (uint64_t)g->npairs * 16, 0u))) ||
         (rc = reset_rounds(g)))
         return rc;
-    // round 0 expands the source over the light CSR
-    if ((rc = launch_seed_csr(g, g->light.off, g->light.col, src, 1u, dist, 1u + delta))) return rc;
+    // round 0 expands the source (over the light CSR when split)
+    const uint32_t thr0 = delta > 0xFFFFFFFEu ? 0xFFFFFFFFu : 1u + delta;
+    if ((rc = launch_seed_csr(g, off, col, src, 1u, dist, thr0))) return rc;
     uint32_t iters = 0;
     uint64_t fsum = 0;
-    RelaxParams p{dist, g->d_wl, g->d_far, delta, 1u, g->heavy.off, g->heavy.col, g->d_wh,
-                  g->heavy.bins.q.with(g->heavy.bins.d_ctr)};
-    if ((rc = run_relax_rounds<true>(g, p, g->light.off, g->light.col, timer, &iters, &fsum))) return rc;
+    RelaxParams p{dist, split ? g->d_wl : g->d_w, g->d_far, delta, split ? 1u : 0u,
+                  split ? g->heavy.off : nullptr, split ? g->heavy.col : nullptr, split ? g->d_wh : nullptr,
+                  split ? g->heavy.bins.q.with(g->heavy.bins.d_ctr) : Frontier{}, defer_deg, window};
+    if ((rc = run_relax_rounds<true>(g, p, off, col, timer, &iters, &fsum))) return rc;
     PTG_HIP(hipEventRecord(g->ev1, g->stream));
     PTG_HIP(hipEventSynchronize(g->ev1));
     if (st) {
@@ -386,7 +470,7 @@ int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uin
     if ((rc = reset_rounds(g))) return rc;
     k_seed_all<<<g->grid, kBlock, 0, g->stream>>>(make_env(g, 0), g->V - g->out.bins.n_zero, g->E);
     PTG_HIP(hipGetLastError());
-    RelaxParams p{label, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}};
+    RelaxParams p{label, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}, 0xFFFFFFFFu, 0u};
     return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum);
 }
 

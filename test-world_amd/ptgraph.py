@@ -74,6 +74,8 @@ class Config(ctypes.Structure):
         ("batch_td_to_bu", ctypes.c_float),
         ("batch_chunk", ctypes.c_uint32),
         ("order_hub_degree", ctypes.c_uint32),
+        ("sssp_defer_degree", ctypes.c_uint32),
+        ("sssp_defer_window", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
     ]
 
@@ -548,7 +550,7 @@ class Graph:
         return c
 
     def tune(self, **changes):
-        """Changes tuning fields (td_to_bu, bu_to_td, sssp_delta, batch_*)."""
+        """Changes tuning fields (td_to_bu, bu_to_td, sssp_*, batch_*)."""
         c = self.get_config()
         for k, v in changes.items():
             setattr(c, k, v)

@@ -197,9 +197,10 @@ def test_autotune_and_set_config():
         want_d, _ = g.sssp_batch(srcs)
         c = g.autotune(srcs)
         got = g.get_config()
-        for f in ("td_to_bu", "bu_to_td", "sssp_delta", "batch_td_to_bu", "batch_grid", "batch_delta",
-                  "batch_chunk"):
+        for f in ("td_to_bu", "bu_to_td", "batch_td_to_bu", "batch_grid", "batch_delta", "batch_chunk"):
             assert getattr(c, f) == getattr(got, f) and getattr(c, f) > 0, f
+        for f in ("sssp_delta", "sssp_defer_degree", "sssp_defer_window"):  # 0 = automatic
+            assert getattr(c, f) == getattr(got, f), f
         say("case autotune result %s" % {f: getattr(c, f) for f, _ in c._fields_})
         # results do not depend on tuning
         np.testing.assert_array_equal(g.bfs_batch(srcs)[0], want_b)

@@ -32,6 +32,10 @@ CONFIGS = {
     "hub_order_big": {"order_hub_degree": 3},
     "delta1": {"sssp_delta": 1},      # light = zero-weight edges only
     "delta1000": {"sssp_delta": 1000},  # every edge light
+    # SSSP deferral: every vertex of degree > 1 waits for a window of 1
+    "defer_all": {"small_degree": 1, "large_degree": 64, "sssp_defer_degree": 1, "sssp_defer_window": 1},
+    "defer_off": {"sssp_defer_degree": 0xFFFFFFFF},
+    "defer_split": {"sssp_delta": 24, "sssp_defer_degree": 8, "sssp_defer_window": 1},
 }
 
 
@@ -310,6 +314,33 @@ def test_sssp_steps_rmat16(rmat16):
                 np.testing.assert_array_equal(d, want[s], err_msg="delta %d src %d" % (delta, s))
     finally:
         gg.tune(sssp_delta=0)
+
+
+def test_sssp_deferral_rmat16(rmat16):
+    """Deferral of high-degree vertices, with and without buckets."""
+    g, gg = rmat16
+    ptr, col, w = g.node_pointers, g.cols, g.weights
+    deg = np.diff(g.offsets.astype(np.int64))
+    srcs = [0] + [int(x) for x in np.random.default_rng(11).choice(np.nonzero(deg > 0)[0], 3, replace=False)]
+    want = {s: oracle.dijkstra(ptr, col, w, s) for s in srcs}
+    rounds = {}
+    try:
+        for delta in (0, 24):
+            for ddeg, win in ((32, 1), (64, 2), (64, 100), (500, 1), (0xFFFFFFFF, 0)):
+                gg.tune(sssp_delta=delta, sssp_defer_degree=ddeg, sssp_defer_window=win)
+                say("case rmat16 sssp delta %d defer degree %d window %d" % (delta, ddeg, win))
+                total = 0
+                for s in srcs:
+                    d, st = gg.sssp(s)
+                    np.testing.assert_array_equal(d, want[s], err_msg="delta %d defer %d/%d src %d" % (
+                        delta, ddeg, win, s))
+                    total += st.iterations
+                rounds[(delta, ddeg, win)] = total
+        say("rounds %s" % rounds)
+        # one unbounded bucket takes fewer rounds than buckets of 24
+        assert rounds[(0, 64, 2)] < rounds[(24, 64, 2)]
+    finally:
+        gg.tune(sssp_delta=0, sssp_defer_degree=0, sssp_defer_window=0)
 
 
 def test_frontier_sizes_match_levels(rmat16):

@@ -53,7 +53,7 @@ struct RoundCtr {
     uint32_t need_heavy;                    // SSSP: the bucket's heavy edges are still to relax
     uint32_t lo;                            // SSSP: lower end of the current bucket
     uint32_t fmin_inv;                      // SSSP: ~(smallest value in this frontier); 0 = unknown
-    uint32_t ndefer;                        // SSSP: entries of the deferred list (pre-pass)
+    uint32_t ndefer;                        // SSSP: 1 if the pre-pass deferred any vertex
     uint32_t pad[5];
     uint32_t found_sh[kShards];             // frontier vertices (any degree)
     uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
@@ -175,10 +175,12 @@ struct ptg_graph {
     uint32_t split_delta = 0;
     ptg::Csr light, heavy;
     uint32_t *d_wl = nullptr, *d_wh = nullptr;
-    // SSSP deferred high-degree vertices of a round (V entries) and the
-    // automatic deferral window (0 = not computed yet).
+    // SSSP deferred high-degree vertices of a round (bitmap of nwords words,
+    // zero between rounds) and the automatic deferral window (0 = not
+    // computed yet).
     uint32_t *d_defer = nullptr;
     uint32_t auto_window = 0;
+    bool defer_dirty = false;  // the deferred bitmap may hold bits of a failed query
 
     // Batched multi-source state (batch.hip), allocated on the first batch.
     ptg::BatchState *batch = nullptr;

@@ -28,9 +28,9 @@
 // at or above the frontier's smallest value plus `window` stays in the
 // frontier without being expanded, so that a hub is expanded close to its
 // final value rather than once per improvement. The pre-pass moves such
-// vertices of flags and bitmap frontiers to a deferred list, which the round
-// puts back into its output frontier; queue frontiers (at most kQueueCap
-// edges) skip them during the walk. The vertex with the smallest value is
+// vertices of flags and bitmap frontiers to a deferred bitmap (whole words,
+// plain stores), which the round puts back into its output frontier and
+// clears; queue frontiers (at most kQueueCap edges) skip them during the walk. The vertex with the smallest value is
 // always expanded, so every round makes progress.
 #pragma once
 
@@ -54,7 +54,7 @@ struct RoundEnv {
     uint32_t sd;                  // small degree threshold
     uint32_t qcap;                // frontier edges up to which the output is queued
     const uint32_t *val;          // SSSP deferral: vertex values
-    uint32_t *dq;                 // SSSP deferral: deferred list of the round
+    uint32_t *dbits;              // SSSP deferral: deferred vertices of the round (bitmap)
     uint32_t defer_deg;           // defer vertices of degree > defer_deg (~0u: never)
     uint32_t window;              // ... whose value >= frontier minimum + window
 };
@@ -359,19 +359,22 @@ __device__ __forceinline__ void expand_frontier(const RoundEnv &e, const RoundCt
 }
 
 // Puts the pre-pass's deferred vertices of round e.r back into the output
-// frontier, a lane per vertex.
+// frontier, a lane per vertex, and clears their bitmap words.
 template <class Op>
 __device__ __forceinline__ void requeue_deferred(const RoundEnv &e, const RoundCtr *in, Op &op, const Emit &em,
                                                  Acc &acc) {
-    const uint32_t n = in->ndefer;
+    if (!in->ndefer) return;
+    const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * kWavesPerBlock;
-    for (uint32_t t = gw * 64; t < n; t += nw * 64) {
-        const uint32_t i = t + lane_id();
-        const bool valid = i < n;
-        const uint32_t v = valid ? e.dq[i] : 0u;
-        const uint32_t d = valid ? e.off[v + 1] - e.off[v] : 0u;
-        op.requeue(valid, v, d, valid ? op.vertex(v) : 0u, em, acc);
+    for (uint32_t p = gw; p < e.npairs; p += nw) {
+        const uint32_t bits = e.dbits[2 * p + (lane >> 5)];
+        const bool mem = (bits >> (lane & 31)) & 1u;
+        if (ballot(mem) == 0) continue;
+        if ((lane & 31) == 0 && bits) e.dbits[2 * p + (lane >> 5)] = 0u;
+        const uint32_t v = p * 64 + lane;
+        const uint32_t d = mem ? e.off[v + 1] - e.off[v] : 0u;
+        op.requeue(mem, v, d, mem ? op.vertex(v) : 0u, em, acc);
     }
 }
 

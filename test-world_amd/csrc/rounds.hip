@@ -33,7 +33,7 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
     e.sd = g->cfg.small_degree;
     e.qcap = kQueueCap;
     e.val = nullptr;
-    e.dq = g->d_defer;
+    e.dbits = g->d_defer;
     e.defer_deg = 0xFFFFFFFFu;
     e.window = 0;
     return e;
@@ -45,7 +45,7 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
 // degree > sd in LDS and appends them to chunk-queue shard b % 16 with one
 // atomic per batch (an LDS scan gives the chunk bases). Members that the
 // round defers (degree > defer_deg, value >= the frontier minimum + window)
-// go to the deferred list instead.
+// go to the deferred bitmap instead.
 __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
     constexpr uint32_t kBuf = 1024;
     __shared__ uint32_t s_v[kBuf], s_n[kBuf];
@@ -90,7 +90,14 @@ __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
         if (lim != 0xFFFFFFFFu) {
             const bool defer = want && d > e.defer_deg && e.val[v] >= lim;
             want = want && !defer;
-            wave_append(e.dq, &in->ndefer, defer, v);
+            const uint64_t md = ballot(defer);
+            if (md) {
+                if ((lane & 31) == 0) {
+                    const uint32_t bits = lane ? (uint32_t)(md >> 32) : (uint32_t)md;
+                    if (bits) e.dbits[2 * p + (lane >> 5)] = bits;
+                }
+                if (lane == 0) in->ndefer = 1u;
+            }
         }
         const uint64_t m = ballot(want);
         if (m) {

@@ -136,7 +136,7 @@ struct BfsParams {
 // One BFS level. Direction (Beamer, Asanovic, Patterson 2012): switch to
 // bottom-up when the frontier's edges exceed the unexplored edges / alpha,
 // back to top-down when the frontier has fewer than V / beta vertices.
-__global__ __launch_bounds__(kBlock) void k_bfs_round(RoundEnv e, BfsParams p) {
+__global__ __launch_bounds__(kBlock, 8) void k_bfs_round(RoundEnv e, BfsParams p) {
     RoundCtr *in = e.ring + e.r % kRing;
     RoundCtr *out = e.ring + (e.r + 1) % kRing;
     clear_ring_ahead(e);

@@ -217,8 +217,27 @@ struct HasDefer<Op, decltype((void)&Op::requeue)> {
 // so a lane keeps kGroup random reads in flight. A value read before another
 // edge of the group is applied can only be larger than the current one
 // (values only fall), so it lets at most more candidates reach the atomic,
-// which decides.
+// which decides. Ops with an issue() step send the group's atomics one after
+// another before any result is used, so their round trips overlap too.
 constexpr uint32_t kGroup = 4;
+
+template <class Op, class = void>
+struct HasIssue {
+    static constexpr bool value = false;
+};
+template <class Op>
+struct HasIssue<Op, decltype((void)&Op::issue)> {
+    static constexpr bool value = true;
+};
+
+// load -> (issue) -> apply over a group of n edges.
+template <class Op, uint32_t n, class Pre>
+__device__ __forceinline__ void issue_group(const Op &op, Pre (&pre)[n]) {
+    if constexpr (HasIssue<Op>::value) {
+#pragma unroll
+        for (uint32_t j = 0; j < n; j++) op.issue(pre[j]);
+    }
+}
 
 // Lane-per-vertex expansion of the tile's vertices of degree 1..sd (`mem`
 // says whether the lane holds a frontier vertex). Wave-uniform call.
@@ -238,6 +257,7 @@ __device__ __forceinline__ void process_tile(const RoundEnv &e, bool mem, uint32
         typename Op::Pre pre[kGroup];
 #pragma unroll
         for (uint32_t j = 0; j < kGroup; j++) pre[j] = op.load(k + j < ds, sv, (uint64_t)b + k + j);
+        issue_group(op, pre);
 #pragma unroll
         for (uint32_t j = 0; j < kGroup; j++)
             if (k + j < md) op.apply(pre[j], em, acc);
@@ -246,7 +266,9 @@ __device__ __forceinline__ void process_tile(const RoundEnv &e, bool mem, uint32
 
 // Walks the chunk queue: wave gw takes the contiguous global chunk range
 // [C*gw/nw, C*(gw+1)/nw), finds its first entry by binary search and then
-// steps through chunks (kWaveChunk edges, 64 lanes per step).
+// steps through chunks (kWaveChunk edges, 64 lanes per step). The entries'
+// ids, offsets and values are loaded 64 at a time, a lane each, and read
+// back with readlane, so a short entry costs no load latency of its own.
 template <class Op>
 __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr *in, Op &op, const Emit &em,
                                               Acc &acc, uint32_t gw, uint32_t nw) {
@@ -270,55 +292,66 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
         if (bases[mid] <= local) lo = mid; else hi = mid;
     }
     uint32_t idx = lo;
-    uint32_t v = ids[idx];
-    uint32_t vb = e.off[v], vd = e.off[v + 1] - vb;
     uint32_t k = local - bases[idx];
-    uint32_t sv = op.vertex(v);
-    bool skip = false;
-    if constexpr (HasDefer<Op>::value) skip = op.defers(vd, __builtin_amdgcn_readfirstlane(sv));
-    for (uint32_t c = c0; c < c1; c++) {
-        while (k >= wave_chunks(vd)) {  // next entry, possibly in the next shard
-            k = 0;
-            if (++idx >= n) {
-                do {
-                    s++;
-                } while (s < kShards - 1 && cpre[s + 1] == cpre[s]);
-                ids = e.cqin + (uint64_t)s * e.cq_cap;
-                n = (uint32_t)(in->cpack[s] >> kLCountShift);
-                idx = 0;
+    uint32_t c = c0;
+    while (c < c1) {
+        // entries [idx, idx + nb) of shard s, one per lane
+        const uint32_t nb = min(64u, n - idx);
+        uint32_t mv = 0, mb = 0, md = 0, ms = 0;
+        if (lane < nb) {
+            mv = ids[idx + lane];
+            mb = e.off[mv];
+            md = e.off[mv + 1] - mb;
+            ms = op.vertex(mv);
+        }
+        for (uint32_t i = 0; i < nb && c < c1; i++) {
+            const uint32_t v = __builtin_amdgcn_readlane(mv, i);
+            const uint32_t vb = __builtin_amdgcn_readlane(mb, i);
+            const uint32_t vd = __builtin_amdgcn_readlane(md, i);
+            const uint32_t sv = __builtin_amdgcn_readlane(ms, i);
+            const uint32_t nch = wave_chunks(vd);
+            bool skip = false;
+            if constexpr (HasDefer<Op>::value) {
+                // A deferred entry (only queue frontiers reach here with one:
+                // the pre-pass filters the others) goes back into the output
+                // frontier once, from the wave that holds its first chunk. A
+                // wave that read an older, larger value and skipped a later
+                // chunk of an entry that another wave expanded left it to the
+                // improvement in between, which put the vertex into the output.
+                skip = op.defers(vd, sv);
+                if (skip) {
+                    if (k == 0) op.requeue(lane == 0, v, vd, sv, em, acc);
+                    const uint32_t left = min(nch - k, c1 - c);
+                    c += left;
+                    k += left;
+                }
             }
-            v = ids[idx];
-            vb = e.off[v];
-            vd = e.off[v + 1] - vb;
-            sv = op.vertex(v);
-            if constexpr (HasDefer<Op>::value) skip = op.defers(vd, __builtin_amdgcn_readfirstlane(sv));
-        }
-        if constexpr (HasDefer<Op>::value) {
-            // A deferred entry (only queue frontiers reach here with one: the
-            // pre-pass filters the others) goes back into the output frontier
-            // once, from the wave that holds its first chunk. A wave that read
-            // an older, larger value and skipped a later chunk of an entry that
-            // another wave expanded left it to the improvement in between,
-            // which put the vertex into the output frontier.
-            if (skip) {
-                if (k == 0) op.requeue(lane == 0, v, vd, __builtin_amdgcn_readfirstlane(sv), em, acc);
-                k++;
-                continue;
+            for (; k < nch && c < c1; k++, c++) {
+                const uint32_t b = k * kWaveChunk;
+                const uint32_t end = min(vd, b + kWaveChunk);
+                static_assert(kWaveChunk == 64 * kGroup, "a chunk is one group of 64-edge steps");
+                typename Op::Pre pre[kGroup];
+#pragma unroll
+                for (uint32_t j = 0; j < kGroup; j++) {
+                    const uint32_t t = b + 64 * j + lane;
+                    pre[j] = op.load(t < end, sv, (uint64_t)vb + t);
+                }
+                issue_group(op, pre);
+#pragma unroll
+                for (uint32_t j = 0; j < kGroup; j++)
+                    if (b + 64 * j < end) op.apply(pre[j], em, acc);
             }
+            if (k >= nch) k = 0;  // the next entry starts at its first chunk
         }
-        const uint32_t b = k * kWaveChunk;
-        const uint32_t end = min(vd, b + kWaveChunk);
-        static_assert(kWaveChunk == 64 * kGroup, "a chunk is one group of 64-edge steps");
-        typename Op::Pre pre[kGroup];
-#pragma unroll
-        for (uint32_t j = 0; j < kGroup; j++) {
-            const uint32_t t = b + 64 * j + lane;
-            pre[j] = op.load(t < end, sv, (uint64_t)vb + t);
+        idx += nb;
+        if (idx >= n && c < c1) {  // on to the next non-empty shard
+            do {
+                s++;
+            } while (s < kShards - 1 && cpre[s + 1] == cpre[s]);
+            ids = e.cqin + (uint64_t)s * e.cq_cap;
+            n = (uint32_t)(in->cpack[s] >> kLCountShift);
+            idx = 0;
         }
-#pragma unroll
-        for (uint32_t j = 0; j < kGroup; j++)
-            if (b + 64 * j < end) op.apply(pre[j], em, acc);
-        k++;
     }
 }
 

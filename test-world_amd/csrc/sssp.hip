@@ -76,7 +76,7 @@ struct Relax {
     }
 
     struct Pre {
-        uint32_t w, nv, cur;
+        uint32_t w, nv, cur, old;
         bool valid;
     };
 
@@ -90,10 +90,18 @@ struct Relax {
         return p;
     }
 
+    // The atomic of a candidate (old = 0 for the others: never improved).
+    __device__ __forceinline__ void issue(Pre &p) const {
+        p.old = p.valid && p.nv < p.cur ? atomicMin(&val[p.w], p.nv) : 0u;
+    }
+
     __device__ __forceinline__ void apply(const Pre &p, const Emit &em, Acc &acc) const {
         const uint32_t w = p.w, nv = p.nv;
-        bool improved = false;
-        if (p.valid && nv < p.cur) improved = nv < atomicMin(&val[w], nv);
+        // Opaque to the optimiser: without it the compare is folded into
+        // issue()'s branch and every atomic is waited for before the next.
+        uint32_t old = p.old;
+        asm volatile("" : "+v"(old));
+        const bool improved = nv < old;
         const bool near = improved && nv < thr;
         if (near) acc.fmin_inv = max(acc.fmin_inv, ~nv);
         if (improved && !near) {
@@ -189,16 +197,18 @@ struct HeavyRelax {
 // edges of the bucket just finished (once per bucket); then split the far set.
 // The far set is non-empty after the round when it was before an expand
 // round, when anything was added or kept, and (conservatively) after a heavy
-// round.
-template <bool kWeighted>
-__global__ __launch_bounds__(kBlock) void k_relax_round(RoundEnv e, RelaxParams p) {
+// round. kBuckets = false (one unbounded bucket, no split: the far set stays
+// empty) compiles the expand round only, with fewer registers. Eight waves
+// per SIMD keep the whole persistent grid (8 workgroups per CU) resident.
+template <bool kWeighted, bool kBuckets>
+__global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxParams p) {
     RoundCtr *in = e.ring + e.r % kRing;
     RoundCtr *out = e.ring + (e.r + 1) % kRing;
     clear_ring_ahead(e);
     const InSummary s = summarize(in);
     const bool expand = s.found > 0;
-    const bool heavy = !expand && p.split && in->need_heavy;
-    if (!expand && !heavy && !in->far_any) return;
+    const bool heavy = kBuckets && !expand && p.split && in->need_heavy;
+    if (!expand && !heavy && !(kBuckets && in->far_any)) return;
     const uint32_t thr = in->thr, lo = in->lo;
     const uint32_t thr2 = thr > 0xFFFFFFFFu - p.delta ? 0xFFFFFFFFu : thr + p.delta;
     const uint32_t mode = !expand ? kModeBitmap : (s.edges <= e.qcap ? kModeQueue : kModeFlags);
@@ -223,10 +233,11 @@ __global__ __launch_bounds__(kBlock) void k_relax_round(RoundEnv e, RelaxParams 
         Relax<kWeighted> op{e.off, e.col, p.wt, p.val, e.fout, e.flout, p.far, thr, p.defer_deg, lim};
         expand_frontier(e, in, op, em, acc);
         if (lim != 0xFFFFFFFFu) requeue_deferred(e, in, op, em, acc);
-    } else if (heavy) {
-        expand_static(p.hbins, p.hoff, HeavyRelax{p.hcol, p.hw, p.val, p.far, lo, thr});
-    } else {
-        sssp_split(e, p.val, p.far, thr, thr2, em, acc);
+    } else if constexpr (kBuckets) {
+        if (heavy)
+            expand_static(p.hbins, p.hoff, HeavyRelax{p.hcol, p.hw, p.val, p.far, lo, thr});
+        else
+            sssp_split(e, p.val, p.far, thr, thr2, em, acc);
     }
     if (ballot(acc.far != 0) && lane_id() == 0) out->far_any = 1u;
     acc.far = 0;
@@ -245,7 +256,10 @@ static int run_relax_rounds(ptg_graph *g, RelaxParams p, const uint32_t *off, co
             env.window = p.window;
             int rc = launch_prepare(g, env);
             if (rc) return rc;
-            timer.launch(k_relax_round<kWeighted>, dim3(g->grid), dim3(kBlock), env, p);
+            if (p.split)
+                timer.launch(k_relax_round<kWeighted, true>, dim3(g->grid), dim3(kBlock), env, p);
+            else
+                timer.launch(k_relax_round<kWeighted, false>, dim3(g->grid), dim3(kBlock), env, p);
             PTG_HIP(hipGetLastError());
             return PTG_OK;
         },

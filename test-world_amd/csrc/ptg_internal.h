@@ -33,7 +33,7 @@ constexpr uint32_t kQueueCap = 16384;   // per-shard queue capacity = max queued
 constexpr uint32_t kModeQueue = 0, kModeBitmap = 1;
 constexpr uint32_t kDirTopDown = 0, kDirBottomUp = 1;
 
-constexpr uint32_t kWaveChunk = 256;   // edges per chunk of the frontier chunk queue
+constexpr uint32_t kWaveChunk = 64;    // edges per chunk of the frontier chunk queue (one wave step)
 
 // Description of the frontier consumed by round r (ring[r % kRing]) and the
 // state carried into that round. Written by round r - 1 (and, for bitmap

@@ -18,8 +18,9 @@
 //             SSSP far-set splits, seeds) and counted exactly; the pre-pass
 //             fills the chunk queue.
 // The chunk queue holds vertices of degree > sd as ceil(d / kWaveChunk)
-// chunks; wave w expands the contiguous chunk range [C*w/W, C*(w+1)/W), so a
-// hub's list is spread over many waves. Degree <= sd vertices are expanded one
+// chunks of 64 edges; wave w expands the contiguous chunk range
+// [C*w/W, C*(w+1)/W), so a hub's list is spread over many waves and every
+// wave gets about the same number of edges. Degree <= sd vertices are expanded one
 // per lane from the S queue or straight from bitmap tiles of 64 vertices.
 // Counters are summed per workgroup and added to 1 of 16 shards (a single hot
 // address serialises atomics: measured 11 ns each on MI355X).
@@ -264,11 +265,26 @@ __device__ __forceinline__ void process_tile(const RoundEnv &e, bool mem, uint32
     }
 }
 
+// Sets the wave's issue priority (0..3; s_setprio takes an immediate).
+__device__ __forceinline__ void set_wave_prio(uint32_t p) {
+    switch (p) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+    }
+}
+
 // Walks the chunk queue: wave gw takes the contiguous global chunk range
 // [C*gw/nw, C*(gw+1)/nw), finds its first entry by binary search and then
-// steps through chunks (kWaveChunk edges, 64 lanes per step). The entries'
+// steps through the chunks (kWaveChunk = 64 edges, a lane each), up to kGroup
+// consecutive chunks of an entry at a time. The entries'
 // ids, offsets and values are loaded 64 at a time, a lane each, and read
 // back with readlane, so a short entry costs no load latency of its own.
+// Equal shares do not finish together: the sequencer issues for the oldest
+// wave first, and later-dispatched waves took up to 2.8x longer per edge. So
+// a wave starts its range at priority 3 and steps down a level per quarter
+// done: waves that are behind are issued for first.
 template <class Op>
 __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr *in, Op &op, const Emit &em,
                                               Acc &acc, uint32_t gw, uint32_t nw) {
@@ -294,6 +310,9 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
     uint32_t idx = lo;
     uint32_t k = local - bases[idx];
     uint32_t c = c0;
+    const uint32_t len = c1 - c0;
+    uint32_t quarter = 0, next_q = c0 + (len + 3) / 4;
+    set_wave_prio(3);
     while (c < c1) {
         // entries [idx, idx + nb) of shard s, one per lane
         const uint32_t nb = min(64u, n - idx);
@@ -326,10 +345,17 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
                     k += left;
                 }
             }
-            for (; k < nch && c < c1; k++, c++) {
+            // up to kGroup consecutive chunks of the entry at once
+            while (k < nch && c < c1) {
+                if (c >= next_q) {
+                    quarter++;
+                    next_q = c0 + (uint32_t)(((uint64_t)len * (quarter + 1) + 3) / 4);
+                    set_wave_prio(3 - min(quarter, 3u));
+                }
+                const uint32_t gn = min(min(kGroup, nch - k), c1 - c);
                 const uint32_t b = k * kWaveChunk;
-                const uint32_t end = min(vd, b + kWaveChunk);
-                static_assert(kWaveChunk == 64 * kGroup, "a chunk is one group of 64-edge steps");
+                const uint32_t end = min(vd, b + kWaveChunk * gn);
+                static_assert(kWaveChunk == 64, "a chunk is one 64-edge wave step");
                 typename Op::Pre pre[kGroup];
 #pragma unroll
                 for (uint32_t j = 0; j < kGroup; j++) {
@@ -340,6 +366,8 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
 #pragma unroll
                 for (uint32_t j = 0; j < kGroup; j++)
                     if (b + 64 * j < end) op.apply(pre[j], em, acc);
+                k += gn;
+                c += gn;
             }
             if (k >= nch) k = 0;  // the next entry starts at its first chunk
         }
@@ -353,6 +381,7 @@ __device__ __forceinline__ void expand_chunks(const RoundEnv &e, const RoundCtr 
             idx = 0;
         }
     }
+    set_wave_prio(0);
 }
 
 // Expands the frontier of round e.r: the chunk queue, then the degree <= sd

@@ -383,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_output(BatchEnv e, uint32_t le
 __device__ __forceinline__ void relax_row(const BatchEnv &e, unsigned long long a, uint32_t dv, uint64_t eb,
                                           uint64_t ee, uint32_t thr, unsigned long long *nxt, uint64_t &t_nf,
                                           uint64_t &t_far) {
-    constexpr uint32_t kU = 8;
+    constexpr uint32_t kU = 4;
     const uint32_t lane = lane_id();
     const bool act = (a >> lane) & 1ull;
     for (uint64_t base = eb; base < ee; base += 64) {
@@ -429,7 +429,7 @@ __device__ __forceinline__ void relax_row(const BatchEnv &e, unsigned long long 
 
 // One SSSP round: expand the frontier masks (near), or, when none is left,
 // raise the threshold by delta and move far bits below it into the frontier.
-__global__ __launch_bounds__(kBlock) void k_mssp_round(BatchEnv e, uint32_t j) {
+__global__ __launch_bounds__(kBlock, 8) void k_mssp_round(BatchEnv e, uint32_t j) {
     const uint32_t R = e.p->base + j;
     BCtr *in = e.ring + R % kBRing;
     BCtr *out = e.ring + (R + 1) % kBRing;

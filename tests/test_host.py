@@ -82,6 +82,20 @@ def test_config_defaults():
     assert c.small_degree < c.large_degree
     assert c.chunk_bytes == 256 << 20
     assert c.order_hub_degree == 16
+    # SSSP: one unbounded bucket with automatic deferral degree and window
+    assert c.sssp_delta == 0 and c.sssp_defer_degree == 0 and c.sssp_defer_window == 0
+
+
+def test_config_layout_matches_header():
+    """The ctypes mirror has the field order and size of ptg_config in ptg.h."""
+    import ctypes
+    import re
+
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "ptg.h")).read()
+    body = hdr[hdr.index("typedef struct ptg_config {"):hdr.index("} ptg_config;")]
+    names = re.findall(r"^\s*(?:uint64_t|uint32_t|float)\s+(\w+);", body, re.M)
+    assert names == [f for f, _ in pg.Config._fields_]
+    assert ctypes.sizeof(pg.Config) % 8 == 0
 
 
 def check_clean_symmetric(g):

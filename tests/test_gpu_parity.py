@@ -343,6 +343,27 @@ def test_sssp_deferral_rmat16(rmat16):
         gg.tune(sssp_delta=0, sssp_defer_degree=0, sssp_defer_window=0)
 
 
+@pytest.mark.parametrize("lo,hi", [(0, 4), (1, 1), (1, 100000), (1000, 1000000)])
+def test_sssp_weight_ranges(lo, hi):
+    """The automatic deferral window follows the mean weight; values stay exact
+    for zero weights, unit weights and weights far above 16 bits."""
+    g = pg.gen_rmat(14, 16, seed=3).add_weights(lo, hi, seed=4)
+    ptr, col, w = g.node_pointers, g.cols, g.weights
+    deg = np.diff(g.offsets.astype(np.int64))
+    srcs = [int(x) for x in np.random.default_rng(5).choice(np.nonzero(deg > 0)[0], 4, replace=False)]
+    want = {s: oracle.dijkstra(ptr, col, w, s) for s in srcs}
+    with pg.Graph.from_csr(g) as gg:
+        for cfg in ({}, {"sssp_defer_degree": 16}):
+            gg.tune(**cfg)
+            for s in srcs:
+                say("case rmat14 weights [%d, %d] %s src %d" % (lo, hi, cfg, s))
+                d, _ = gg.sssp(s)
+                np.testing.assert_array_equal(d, want[s], err_msg="weights [%d, %d] %s src %d" % (lo, hi, cfg, s))
+        rows, _ = gg.sssp_batch(np.array(srcs, dtype=np.uint32))
+        for k, s in enumerate(srcs):
+            np.testing.assert_array_equal(rows[k], want[s], err_msg="batch weights [%d, %d] src %d" % (lo, hi, s))
+
+
 def test_frontier_sizes_match_levels(rmat16):
     g, gg = rmat16
     say("case frontier sizes")

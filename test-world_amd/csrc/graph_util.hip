@@ -130,11 +130,28 @@ __global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32
     const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     uint64_t cnt = 0, edges = 0;
-    for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64) {
-        const uint64_t v = base + lane;
-        if (v < V && val[v] != unreached) {
-            cnt++;
-            edges += off[v + 1] - off[v];
+    // four 64-vertex tiles per step; values and offsets are loaded together
+    // (lane l's next offset is lane l+1's; lane 63 and vertex V-1 load their own)
+    constexpr uint32_t kU = 4;
+    for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64 * kU) {
+        uint32_t x[kU], o[kU], o63[kU];
+#pragma unroll
+        for (uint32_t j = 0; j < kU; j++) {
+            const uint64_t v = base + (uint64_t)j * nw * 64 + lane;
+            x[j] = v < V ? val[v] : unreached;
+            o[j] = v < V ? off[v] : 0u;
+            o63[j] = v < V && (lane == 63 || v + 1 == V) ? off[v + 1] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kU; j++) {
+            const uint32_t up = (uint32_t)__shfl_down((int)o[j], 1, 64);
+            const uint64_t v = base + (uint64_t)j * nw * 64 + lane;
+            const uint32_t next = lane == 63 || v + 1 == V ? o63[j] : up;
+            // lanes past V hold unreached, so their (garbage) next is unused
+            if (v < V && x[j] != unreached) {
+                cnt++;
+                edges += next - o[j];
+            }
         }
     }
     cnt = wave_sum_u64(cnt);

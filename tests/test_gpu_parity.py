@@ -51,6 +51,12 @@ def device():
         oracle.build()
 
 
+def ptr_degrees(ptr, E):
+    """Out-degrees from V reference-layout offsets (the last one runs to E)."""
+    p = np.append(np.asarray(ptr, dtype=np.int64), E)
+    return np.diff(p)
+
+
 def load(path):
     with np.load(path, allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
@@ -92,8 +98,12 @@ def test_fixture_parity(path, cfg, rows):
             reached = f["bfs"][i] != 0xFFFFFFFF
             assert st.reached == int(reached.sum())
             assert st.iterations == int(f["bfs"][i][reached].max())
-            d, _ = g.sssp(int(s))
+            d, st = g.sssp(int(s))
             np.testing.assert_array_equal(d, f["sssp"][i], err_msg="sssp src %d" % s)
+            # reach statistics (a scan over V that need not be a multiple of 64)
+            r_ss = f["sssp"][i] != oracle.SSSP_SENTINEL_SHARE
+            assert st.reached == int(r_ss.sum())
+            assert st.edges_traversed == int(ptr_degrees(f["ptr"], f["col"].size)[r_ss].sum())
         lab, _ = g.cc()
         np.testing.assert_array_equal(lab, f["cc"])
         r, _ = g.pagerank(0.85, 20)

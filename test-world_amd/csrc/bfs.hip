@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_bfs_round(RoundEnv e, BfsParams p
     uint32_t dir = in->dir;
     if (p.symmetric) {
         if (dir == kDirTopDown && (double)s.edges > m_u / p.alpha) dir = kDirBottomUp;
-        else if (dir == kDirBottomUp && (double)s.found < (double)e.V / p.beta) dir = kDirTopDown;
+        else if (dir == kDirBottomUp && bfs_leaves_bottom_up(s.found, e.V, p.beta)) dir = kDirTopDown;
     }
     const uint32_t mode = dir == kDirBottomUp ? kModeBitmap : (s.edges <= e.qcap ? kModeQueue : kModeFlags);
     clear_bitmap_ahead(e);
@@ -182,14 +182,15 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
     rc = drive_rounds(
         g,
         [&](uint32_t r) -> int {
-            const RoundEnv env = make_env(g, r);
+            RoundEnv env = make_env(g, r);
+            env.bu_beta = p.beta;
             int rc2 = launch_prepare(g, env);
             if (rc2) return rc2;
             timer.launch(k_bfs_round, dim3(g->grid), dim3(kBlock), env, p);
             PTG_HIP(hipGetLastError());
             return PTG_OK;
         },
-        [](const RoundCtr &c) { return c.any == 0; }, &iters, &reached, &edges);
+        [](const RoundCtr &c) { return c.any == 0; }, g->last_rounds[0], &iters, &reached, &edges);
     if (rc) return rc;
     PTG_HIP(hipEventRecord(g->ev1, g->stream));
     PTG_HIP(hipEventSynchronize(g->ev1));

@@ -189,6 +189,10 @@ struct ptg_graph {
 
     // Frontier size of every round of the last BFS / SSSP / label-propagation query.
     std::vector<uint64_t> frontiers;
+    // Rounds of the last BFS [0], SSSP [1] and label-propagation [2] query
+    // (0 = none yet): the next query of the kind launches that many rounds + 1
+    // before its first readback.
+    uint32_t last_rounds[3] = {0, 0, 0};
 
     int symmetric = -1;  // -1 unknown, 0/1 once checked
     uint64_t device_bytes = 0;                          // charged against the budget

@@ -58,6 +58,7 @@ struct RoundEnv {
     uint32_t *dbits;              // SSSP deferral: deferred vertices of the round (bitmap)
     uint32_t defer_deg;           // defer vertices of degree > defer_deg (~0u: never)
     uint32_t window;              // ... whose value >= frontier minimum + window
+    float bu_beta;                // BFS: bottom-up -> top-down switch factor (0: not BFS)
 };
 
 // Saturating a + b.
@@ -153,6 +154,13 @@ __device__ __forceinline__ InSummary summarize(const RoundCtr *in) {
         s.edges += in->edges_sh[i];
     }
     return s;
+}
+
+// BFS: a bottom-up round hands back to top-down once its output frontier
+// has fewer than V / beta vertices (Beamer, Asanovic, Patterson 2012). Both
+// the pre-pass and the round decide with this one function.
+__device__ __forceinline__ bool bfs_leaves_bottom_up(uint32_t found, uint32_t V, float beta) {
+    return (double)found < (double)V / beta;
 }
 
 // Zeroes the ring entry two rounds ahead (done by every round, active or
@@ -476,9 +484,13 @@ inline HostSummary host_summary(const RoundCtr &c) {
 // flag, so idleness is judged on every entry but counts on complete ones.
 // Returns the number of rounds that did work and the sums of their counts.
 template <class Launch, class Idle>
-int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t *rounds_out, uint64_t *found_sum,
+int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_t *rounds_out, uint64_t *found_sum,
                  uint64_t *edges_sum) {
-    uint32_t r = 0, batch = 2, rounds = 0;
+    // The first batch covers the previous query's rounds + 1 (one readback when
+    // the depth repeats); later batches start at 4 and double up to 16. Rounds
+    // launched past the end find an idle entry and return at once.
+    uint32_t r = 0, batch = hint ? std::min(hint + 1, 64u) : 2u, rounds = 0;
+    const bool hinted = hint != 0;
     uint64_t fs = 0, es = 0;
     uint32_t checked = 0;  // ring entries [0, checked) already summed
     g->frontiers.clear();
@@ -504,8 +516,9 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t *rounds_out, u
             rounds++;
         }
         if (done) break;
-        if (batch < 16) batch *= 2;
+        batch = hinted && r <= hint + 1 ? 4u : std::min(batch * 2, 16u);
     }
+    hint = rounds;
     *rounds_out = rounds;
     if (found_sum) *found_sum = fs;
     if (edges_sum) *edges_sum = es;

@@ -36,6 +36,7 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
     e.dbits = g->d_defer;
     e.defer_deg = 0xFFFFFFFFu;
     e.window = 0;
+    e.bu_beta = 0.0f;
     return e;
 }
 
@@ -45,7 +46,8 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
 // degree > sd in LDS and appends them to chunk-queue shard b % 16 with one
 // atomic per batch (an LDS scan gives the chunk bases). Members that the
 // round defers (degree > defer_deg, value >= the frontier minimum + window)
-// go to the deferred bitmap instead.
+// go to the deferred bitmap instead. A BFS round that stays bottom-up reads
+// the bitmap alone, so its pre-pass has nothing to do.
 __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
     constexpr uint32_t kBuf = 1024;
     __shared__ uint32_t s_v[kBuf], s_n[kBuf];
@@ -54,6 +56,9 @@ __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
     RoundCtr *in = e.ring + e.r % kRing;
     const uint32_t mode = in->mode;
     if (mode == kModeQueue) return;
+    if (e.bu_beta > 0.0f && mode == kModeBitmap && in->dir == kDirBottomUp &&
+        !bfs_leaves_bottom_up(summarize(in).found, e.V, e.bu_beta))
+        return;
     const bool from_flags = mode == kModeFlags;
     const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
     const uint32_t T = (e.npairs + gridDim.x - 1) / gridDim.x;

@@ -264,7 +264,8 @@ static int run_relax_rounds(ptg_graph *g, RelaxParams p, const uint32_t *off, co
             PTG_HIP(hipGetLastError());
             return PTG_OK;
         },
-        [](const RoundCtr &c) { return c.any == 0 && c.far_any == 0 && c.need_heavy == 0; }, iters, fsum, nullptr);
+        [](const RoundCtr &c) { return c.any == 0 && c.far_any == 0 && c.need_heavy == 0; },
+        g->last_rounds[kWeighted ? 1 : 2], iters, fsum, nullptr);
 }
 
 // Counts the light edges (weight < delta) of every vertex: one wave each.

@@ -373,56 +373,68 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_output(BatchEnv e, uint32_t le
 // ---------------------------------------------------------------------------
 
 // Relaxes the edges [eb, ee) of vertex v for every source whose bit is in
-// `a` (lane k = source k; dv = this lane's distance of v). Rows are read
-// kU edges at a time, one group ahead of the group being relaxed, so a
-// wave keeps up to 2 kU rows in flight. The atomics return nothing the
-// wave waits for: an improvement is taken from the row just read (values
-// only fall, so every thread that lowers dist[w][k] sees nd < old and sets
-// the bit; a bit set by a thread another one beat only re-expands w with its
-// current value).
+// `a` (lane k = source k; dv = this lane's distance of v). Rows are read in
+// groups of kP edges, the next group's loads issued before the current
+// group's atomics. Every lane issues every load (lanes of sources not in `a`
+// read the first active source's word of the same row, so no line is added),
+// which keeps the loads out of divergent branches: the compiler can then
+// count them and wait for one group while the next is in flight. The atomics
+// return nothing the wave waits for: an improvement is taken from the row
+// just read (values only fall, so every thread that lowers dist[w][k] sees
+// nd < old and sets the bit; a bit set by a thread another one beat only
+// re-expands w with its current value).
 __device__ __forceinline__ void relax_row(const BatchEnv &e, unsigned long long a, uint32_t dv, uint64_t eb,
                                           uint64_t ee, uint32_t thr, unsigned long long *nxt, uint64_t &t_nf,
                                           uint64_t &t_far) {
-    constexpr uint32_t kU = 4;
+    constexpr uint32_t kP = 4;  // 8 keeps more loads in flight but spills at 64 VGPRs
     const uint32_t lane = lane_id();
     const bool act = (a >> lane) & 1ull;
+    const uint32_t sl = act ? lane : (uint32_t)__builtin_ctzll(a);
     for (uint64_t base = eb; base < ee; base += 64) {
         const uint32_t cnt = (uint32_t)min((uint64_t)64, ee - base);
         const uint32_t mw = lane < cnt ? e.col[base + lane] : 0u;
         const uint32_t mwt = lane < cnt ? e.wt[base + lane] : 0u;
-        uint32_t w[kU], old[kU];
+        // row words of edges q .. q + kP - 1 (indexes past cnt - 1 repeat the last edge)
+        auto load = [&](uint32_t q, uint32_t (&old)[kP]) {
 #pragma unroll
-        for (uint32_t t = 0; t < kU; t++) {
-            w[t] = read_lane(mw, min(t, cnt - 1));
-            old[t] = (act && t < cnt) ? e.dist[(uint64_t)w[t] * 64 + lane] : 0u;
-        }
-        for (uint32_t q = 0; q < cnt; q += kU) {
-            uint32_t w2[kU], old2[kU];
+            for (uint32_t t = 0; t < kP; t++)
+                old[t] = e.dist[(uint64_t)read_lane(mw, min(q + t, cnt - 1)) * 64 + sl];
+        };
+        auto relax = [&](uint32_t q, const uint32_t (&old)[kP]) {
 #pragma unroll
-            for (uint32_t t = 0; t < kU; t++) {
-                const uint32_t qq = q + kU + t;
-                w2[t] = read_lane(mw, min(qq, cnt - 1));
-                old2[t] = (act && qq < cnt) ? e.dist[(uint64_t)w2[t] * 64 + lane] : 0u;
-            }
-#pragma unroll
-            for (uint32_t t = 0; t < kU; t++) {
-                const uint32_t nd = dv + read_lane(mwt, min(q + t, cnt - 1));
+            for (uint32_t t = 0; t < kP; t++) {
+                const uint32_t i = min(q + t, cnt - 1);
+                const uint32_t w = read_lane(mw, i);
+                const uint32_t nd = dv + read_lane(mwt, i);
                 const bool imp = act && q + t < cnt && nd < old[t];
-                if (imp) atomicMin(&e.dist[(uint64_t)w[t] * 64 + lane], nd);
+                if (imp) atomicMin(&e.dist[(uint64_t)w * 64 + lane], nd);
                 const unsigned long long mn = ballot(imp && nd < thr);
                 const unsigned long long mf = ballot(imp && nd >= thr);
                 if (lane == 0) {
-                    if (mn) atomicOr(&nxt[w[t]], mn);
-                    if (mf) atomicOr(&e.far[w[t]], mf);
+                    if (mn) atomicOr(&nxt[w], mn);
+                    if (mf) atomicOr(&e.far[w], mf);
                 }
                 t_nf |= mn != 0;
                 t_far |= mf != 0;
             }
-#pragma unroll
-            for (uint32_t t = 0; t < kU; t++) {
-                w[t] = w2[t];
-                old[t] = old2[t];
+        };
+        // two buffers in turn, so no loaded value is copied (a copy would
+        // wait for its load)
+        uint32_t ra[kP], rb[kP];
+        load(0, ra);
+        for (uint32_t q = 0;; q += 2 * kP) {
+            if (q + kP >= cnt) {
+                relax(q, ra);
+                break;
             }
+            load(q + kP, rb);
+            relax(q, ra);
+            if (q + 2 * kP >= cnt) {
+                relax(q + kP, rb);
+                break;
+            }
+            load(q + 2 * kP, ra);
+            relax(q + kP, rb);
         }
     }
 }

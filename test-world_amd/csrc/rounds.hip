@@ -40,6 +40,146 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
     return e;
 }
 
+// LDS staging of the pre-pass: chunk-queue entries of the workgroup,
+// flushed to its shard with one atomic per batch.
+struct PrepLds {
+    static constexpr uint32_t kU = 4;                       // tiles per wave per step
+    static constexpr uint32_t kBuf = 2048;                  // entries; one step adds <= kBlock * kU
+    static constexpr uint32_t kFlushAt = kBuf - kBlock * kU;
+    static constexpr uint32_t kPer = kBuf / kBlock;         // entries per thread in the flush scan
+    uint32_t v[kBuf], n[kBuf];
+    uint32_t cnt, wsum[kWavesPerBlock];
+    unsigned long long old;
+};
+
+// Tiles [t_begin, t_end) of the workgroup, kU per wave per step. Every load
+// of a step (the frontier word or flag byte, offsets v and v + 1) is issued
+// before any is used and none is skipped by a lane (indexes are clamped into
+// range), so one memory latency covers kWavesPerBlock * kU tiles.
+template <bool kFlags>
+__device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, uint32_t lim, PrepLds &L) {
+    constexpr uint32_t kU = PrepLds::kU;
+    constexpr uint32_t kStep = kWavesPerBlock * kU;
+    const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
+    const uint32_t T = (e.npairs + gridDim.x - 1) / gridDim.x;
+    const uint32_t t_begin = blockIdx.x * T;
+    const uint32_t t_end = min(e.npairs, t_begin + T);
+    const uint32_t shard = blockIdx.x % kShards;
+    uint32_t *ids = e.cqin + (uint64_t)shard * e.cq_cap;
+    uint32_t *bases = e.cbin + (uint64_t)shard * e.cq_cap;
+    Acc acc;
+    if (threadIdx.x == 0) L.cnt = 0;
+    __syncthreads();
+    for (uint32_t t0 = t_begin; t0 < t_end; t0 += kStep) {
+        uint32_t x[kU], o0[kU], o1[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t p = min(t0 + u * kWavesPerBlock + wib, t_end - 1);
+            const uint32_t v = p * 64 + lane;
+            if constexpr (kFlags)
+                x[u] = e.flin[v];  // the flag array covers npairs * 64 bytes
+            else
+                x[u] = e.fin[2 * p + (lane >> 5)];
+            o0[u] = e.off[min(v, e.V)];
+            o1[u] = e.off[min(v + 1, e.V)];
+        }
+        uint32_t d[kU];
+        bool want[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t p = t0 + u * kWavesPerBlock + wib;
+            const bool ok = p < t_end;
+            bool mem;
+            if constexpr (kFlags) {
+                mem = ok && x[u] != 0;
+                if (mem) e.flin[p * 64 + lane] = 0;
+                const uint64_t m = ballot(mem);
+                if (ok && (lane & 31) == 0) e.fin[2 * p + (lane >> 5)] = lane ? (uint32_t)(m >> 32) : (uint32_t)m;
+            } else {
+                mem = ok && ((x[u] >> (lane & 31)) & 1u);
+            }
+            d[u] = mem ? o1[u] - o0[u] : 0u;
+            if (kFlags && mem) {
+                acc.found++;
+                acc.edges += d[u];
+            }
+            want[u] = mem && d[u] > e.sd;
+        }
+        if (lim != 0xFFFFFFFFu) {
+            // SSSP deferral: heavy members at or above the limit wait in dbits
+            uint32_t val[kU];
+#pragma unroll
+            for (uint32_t u = 0; u < kU; u++)
+                val[u] = want[u] && d[u] > e.defer_deg ? e.val[(t0 + u * kWavesPerBlock + wib) * 64 + lane] : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < kU; u++) {
+                const uint32_t p = t0 + u * kWavesPerBlock + wib;
+                const bool defer = want[u] && d[u] > e.defer_deg && val[u] >= lim;
+                want[u] = want[u] && !defer;
+                const uint64_t md = ballot(defer);
+                if (md) {
+                    if ((lane & 31) == 0) {
+                        const uint32_t bits = lane ? (uint32_t)(md >> 32) : (uint32_t)md;
+                        if (bits) e.dbits[2 * p + (lane >> 5)] = bits;
+                    }
+                    if (lane == 0) in->ndefer = 1u;
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint64_t m = ballot(want[u]);
+            if (m) {
+                uint32_t base = 0;
+                if (lane == lowest_lane(m)) base = atomicAdd(&L.cnt, (uint32_t)__popcll(m));
+                base = read_lane(base, lowest_lane(m));
+                if (want[u]) {
+                    L.v[base + rank_below(m)] = (t0 + u * kWavesPerBlock + wib) * 64 + lane;
+                    L.n[base + rank_below(m)] = wave_chunks(d[u]);
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t cnt = L.cnt;
+        if (cnt > PrepLds::kFlushAt || t0 + kStep >= t_end) {
+            // exclusive scan of chunk counts, kPer entries per thread
+            uint32_t loc[PrepLds::kPer], sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < PrepLds::kPer; j++) {
+                const uint32_t i = threadIdx.x * PrepLds::kPer + j;
+                loc[j] = i < cnt ? L.n[i] : 0u;
+                sum += loc[j];
+            }
+            const uint32_t incl = wave_incl_scan_u32(sum);
+            if (lane == 63) L.wsum[wib] = incl;
+            __syncthreads();
+            uint32_t woff = 0, total = 0;
+            for (int w = 0; w < kWavesPerBlock; w++) {
+                if (w < (int)wib) woff += L.wsum[w];
+                total += L.wsum[w];
+            }
+            if (threadIdx.x == 0 && cnt)
+                L.old = atomicAdd(&in->cpack[shard], ((unsigned long long)cnt << kLCountShift) | total);
+            __syncthreads();
+            const uint32_t slot0 = (uint32_t)(L.old >> kLCountShift);
+            uint32_t run = (uint32_t)(L.old & kLChunkMask) + woff + (incl - sum);
+#pragma unroll
+            for (uint32_t j = 0; j < PrepLds::kPer; j++) {
+                const uint32_t i = threadIdx.x * PrepLds::kPer + j;
+                if (i < cnt) {
+                    ids[slot0 + i] = L.v[i];
+                    bases[slot0 + i] = run;
+                }
+                run += loc[j];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) L.cnt = 0;
+            __syncthreads();
+        }
+    }
+    if (kFlags) add_tallies(in, acc);
+}
+
 // Pre-pass of round e.r for flags and bitmap frontiers. Workgroup b handles
 // tiles [b*T, (b+1)*T): for a flags frontier it packs (and clears) the bytes
 // into bitmap words and counts the frontier; for both it gathers members of
@@ -49,110 +189,18 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
 // go to the deferred bitmap instead. A BFS round that stays bottom-up reads
 // the bitmap alone, so its pre-pass has nothing to do.
 __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
-    constexpr uint32_t kBuf = 1024;
-    __shared__ uint32_t s_v[kBuf], s_n[kBuf];
-    __shared__ uint32_t s_cnt, s_wsum[kWavesPerBlock];
-    __shared__ unsigned long long s_old;
+    __shared__ PrepLds L;
     RoundCtr *in = e.ring + e.r % kRing;
     const uint32_t mode = in->mode;
     if (mode == kModeQueue) return;
     if (e.bu_beta > 0.0f && mode == kModeBitmap && in->dir == kDirBottomUp &&
         !bfs_leaves_bottom_up(summarize(in).found, e.V, e.bu_beta))
         return;
-    const bool from_flags = mode == kModeFlags;
-    const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
-    const uint32_t T = (e.npairs + gridDim.x - 1) / gridDim.x;
-    const uint32_t t_begin = blockIdx.x * T;
-    const uint32_t t_end = min(e.npairs, t_begin + T);
-    const uint32_t shard = blockIdx.x % kShards;
     const uint32_t lim = e.defer_deg != 0xFFFFFFFFu ? defer_limit(in, e.window) : 0xFFFFFFFFu;
-    uint32_t *ids = e.cqin + (uint64_t)shard * e.cq_cap;
-    uint32_t *bases = e.cbin + (uint64_t)shard * e.cq_cap;
-    Acc acc;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    for (uint32_t t0 = t_begin; t0 < t_end; t0 += kWavesPerBlock) {
-        const uint32_t p = t0 + wib;
-        const uint32_t v = p * 64 + lane;
-        bool mem = false;
-        uint32_t d = 0;
-        if (p < t_end) {
-            if (from_flags) {
-                mem = e.flin[v] != 0;  // the flag array covers npairs * 64 bytes
-                if (mem) e.flin[v] = 0;
-                const uint64_t m = ballot(mem);
-                if ((lane & 31) == 0) e.fin[2 * p + (lane >> 5)] = lane ? (uint32_t)(m >> 32) : (uint32_t)m;
-            } else {
-                mem = (e.fin[2 * p + (lane >> 5)] >> (lane & 31)) & 1u;
-            }
-            if (mem) d = e.off[v + 1] - e.off[v];
-        }
-        if (from_flags && mem) {
-            acc.found++;
-            acc.edges += d;
-        }
-        bool want = mem && d > e.sd;
-        if (lim != 0xFFFFFFFFu) {
-            const bool defer = want && d > e.defer_deg && e.val[v] >= lim;
-            want = want && !defer;
-            const uint64_t md = ballot(defer);
-            if (md) {
-                if ((lane & 31) == 0) {
-                    const uint32_t bits = lane ? (uint32_t)(md >> 32) : (uint32_t)md;
-                    if (bits) e.dbits[2 * p + (lane >> 5)] = bits;
-                }
-                if (lane == 0) in->ndefer = 1u;
-            }
-        }
-        const uint64_t m = ballot(want);
-        if (m) {
-            uint32_t base = 0;
-            if (lane == lowest_lane(m)) base = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
-            base = read_lane(base, lowest_lane(m));
-            if (want) {
-                s_v[base + rank_below(m)] = v;
-                s_n[base + rank_below(m)] = wave_chunks(d);
-            }
-        }
-        __syncthreads();
-        const uint32_t cnt = s_cnt;
-        if (cnt > kBuf - kBlock || t0 + kWavesPerBlock >= t_end) {
-            // exclusive scan of chunk counts, 4 entries per thread
-            uint32_t loc[4], sum = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t i = threadIdx.x * 4 + j;
-                loc[j] = i < cnt ? s_n[i] : 0u;
-                sum += loc[j];
-            }
-            const uint32_t incl = wave_incl_scan_u32(sum);
-            if (lane == 63) s_wsum[wib] = incl;
-            __syncthreads();
-            uint32_t woff = 0, total = 0;
-            for (int w = 0; w < kWavesPerBlock; w++) {
-                if (w < (int)wib) woff += s_wsum[w];
-                total += s_wsum[w];
-            }
-            if (threadIdx.x == 0 && cnt)
-                s_old = atomicAdd(&in->cpack[shard], ((unsigned long long)cnt << kLCountShift) | total);
-            __syncthreads();
-            const uint32_t slot0 = (uint32_t)(s_old >> kLCountShift);
-            uint32_t run = (uint32_t)(s_old & kLChunkMask) + woff + (incl - sum);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t i = threadIdx.x * 4 + j;
-                if (i < cnt) {
-                    ids[slot0 + i] = s_v[i];
-                    bases[slot0 + i] = run;
-                }
-                run += loc[j];
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) s_cnt = 0;
-            __syncthreads();
-        }
-    }
-    if (from_flags) add_tallies(in, acc);
+    if (mode == kModeFlags)
+        prepare_tiles<true>(e, in, lim, L);
+    else
+        prepare_tiles<false>(e, in, lim, L);
 }
 
 // Seeds round 0 with a single source (value src_value) in queue mode.

@@ -173,7 +173,7 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
     if (rc) return rc;
     LaunchTimer timer(g);
     PTG_HIP(hipEventRecord(g->ev0, g->stream));
-    if ((rc = fill_u32(g, level, g->V, kUnreached)) || (rc = reset_rounds(g))) return rc;
+    if ((rc = reset_rounds(g, {FillSpan{level, g->V, kUnreached}}))) return rc;
     if ((rc = launch_seed(g, src, 1u, level, 0u))) return rc;
     const BfsParams p{level, g->E, g->cfg.td_to_bu > 0 ? g->cfg.td_to_bu : 14.0f,
                       g->cfg.bu_to_td > 0 ? g->cfg.bu_to_td : 24.0f, sym};
@@ -193,7 +193,7 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
         [](const RoundCtr &c) { return c.any == 0; }, g->last_rounds[0], &iters, &reached, &edges);
     if (rc) return rc;
     PTG_HIP(hipEventRecord(g->ev1, g->stream));
-    PTG_HIP(hipEventSynchronize(g->ev1));
+    if ((rc = finish_query(g))) return rc;
     if (st) {
         memset(st, 0, sizeof *st);
         float ms = 0;

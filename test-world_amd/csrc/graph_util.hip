@@ -19,6 +19,46 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, u
     for (uint64_t j = n4 * 4 + i; j < n; j += stride) p[j] = v;
 }
 
+struct FillSet {
+    FillSpan s[kMaxSpans];
+    uint32_t k;
+};
+
+// Fills every span of the set: uint4 stores over each span's 16-byte-aligned
+// body, words for its head and tail.
+__global__ __launch_bounds__(kBlock) void k_fill_spans(FillSet f) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t s = 0; s < f.k; s++) {
+        uint32_t *p = f.s[s].p;
+        const uint64_t n = f.s[s].n;
+        const uint32_t v = f.s[s].v;
+        const uint64_t head = min(n, (uint64_t)((16u - ((uint32_t)(uintptr_t)p & 15u)) & 15u) / 4);
+        const uint64_t n4 = (n - head) / 4;
+        uint4 *p4 = reinterpret_cast<uint4 *>(p + head);
+        const uint4 vv = make_uint4(v, v, v, v);
+        if (i < head) p[i] = v;
+        for (uint64_t j = i; j < n4; j += stride) p4[j] = vv;
+        for (uint64_t j = head + n4 * 4 + i; j < n; j += stride) p[j] = v;
+    }
+}
+
+int fill_spans(ptg_graph *g, const FillSpan *spans, uint32_t k) {
+    if (k > kMaxSpans) return fail(PTG_ERR_INVALID_ARG, "fill_spans: too many spans");
+    FillSet f{};
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < k; i++)
+        if (spans[i].n) {
+            f.s[f.k++] = spans[i];
+            total += spans[i].n;
+        }
+    if (!f.k) return PTG_OK;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total / 4 + kBlock - 1) / kBlock + 1, g->grid);
+    k_fill_spans<<<blocks, kBlock, 0, g->stream>>>(f);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
 int fill_u32(ptg_graph *g, uint32_t *p, uint64_t n, uint32_t v) {
     if (n == 0) return PTG_OK;
     const bool vec = ((uintptr_t)p & 15u) == 0;
@@ -175,16 +215,16 @@ __global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32
     }
 }
 
-int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached, uint64_t *edges) {
-    if (!reached && !edges) return PTG_OK;
-    unsigned long long *acc = g->d_acc;
-    int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 4 * kShards, 0u);
-    if (rc) return rc;
-    k_reach_stats<<<g->grid, kBlock, 0, g->stream>>>(g->V, val, unreached, g->out.off, acc);
+FillSpan reach_reset_span(ptg_graph *g) { return FillSpan{reinterpret_cast<uint32_t *>(g->d_acc), 4 * kShards, 0u}; }
+
+static int reach_count(ptg_graph *g, const uint32_t *val, uint32_t unreached, unsigned long long *h) {
+    k_reach_stats<<<g->grid, kBlock, 0, g->stream>>>(g->V, val, unreached, g->out.off, g->d_acc);
     PTG_HIP(hipGetLastError());
-    unsigned long long h[2 * kShards];
-    PTG_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, g->stream));
-    PTG_HIP(hipStreamSynchronize(g->stream));
+    PTG_HIP(hipMemcpyAsync(h, g->d_acc, sizeof(unsigned long long) * 2 * kShards, hipMemcpyDeviceToHost, g->stream));
+    return PTG_OK;
+}
+
+static void reach_sum(const unsigned long long *h, uint32_t *reached, uint64_t *edges) {
     uint64_t c = 0, e = 0;
     for (int s = 0; s < kShards; s++) {
         c += h[2 * s];
@@ -192,6 +232,34 @@ int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t 
     }
     if (reached) *reached = (uint32_t)c;
     if (edges) *edges = e;
+}
+
+// The in-query form reads back into the pinned h_acc (in-memory graphs).
+int reach_stats_launch(ptg_graph *g, const uint32_t *val, uint32_t unreached) {
+    if (!g->h_acc) return fail(PTG_ERR_UNSUPPORTED, "reach_stats_launch: no pinned accumulators");
+    return reach_count(g, val, unreached, g->h_acc);
+}
+
+void reach_stats_read(const ptg_graph *g, uint32_t *reached, uint64_t *edges) { reach_sum(g->h_acc, reached, edges); }
+
+int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached, uint64_t *edges) {
+    if (!reached && !edges) return PTG_OK;
+    const FillSpan z = reach_reset_span(g);
+    unsigned long long h[2 * kShards];
+    int rc;
+    if ((rc = fill_spans(g, &z, 1)) || (rc = reach_count(g, val, unreached, h))) return rc;
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    reach_sum(h, reached, edges);
+    return PTG_OK;
+}
+
+int finish_query(ptg_graph *g) {
+    if (g->deliver) {
+        int rc = g->deliver();
+        if (rc) return rc;
+        g->delivered = true;
+    }
+    PTG_HIP(hipStreamSynchronize(g->stream));
     return PTG_OK;
 }
 

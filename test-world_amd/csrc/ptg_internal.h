@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -152,6 +153,12 @@ struct ptg_graph {
     ptg::RoundCtr *d_ring = nullptr;    // kRing per-round counter blocks
     ptg::RoundCtr *h_ring = nullptr;    // pinned mirror
     unsigned long long *d_acc = nullptr; // 2 x kShards accumulators for statistics kernels
+    unsigned long long *h_acc = nullptr; // pinned mirror of d_acc
+    // Output delivery of the running query (unpermute and/or copy to the
+    // caller's buffer), registered by the entry point; a query that ends in
+    // finish_query() enqueues it before its one final wait.
+    std::function<int()> deliver;
+    bool delivered = false;
 
     // PageRank state.
     float *d_contrib[2] = {};
@@ -229,6 +236,22 @@ int check_symmetric(ptg_graph *g, int *sym);
 int ensure_in_edges(ptg_graph *g);
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,
                 uint64_t *edges);
+// The same in three steps inside a query: zero d_acc (reach_reset_span, part
+// of the query's reset), enqueue the count and its readback, and read the
+// pinned copy after the stream has been waited for.
+struct FillSpan {
+    uint32_t *p;
+    uint64_t n;
+    uint32_t v;
+};
+FillSpan reach_reset_span(ptg_graph *g);
+int reach_stats_launch(ptg_graph *g, const uint32_t *val, uint32_t unreached);
+void reach_stats_read(const ptg_graph *g, uint32_t *reached, uint64_t *edges);
+// Fills up to kMaxSpans word ranges with one launch.
+constexpr uint32_t kMaxSpans = 10;
+int fill_spans(ptg_graph *g, const FillSpan *spans, uint32_t k);
+// Enqueues the registered output delivery, if any, then waits for the stream.
+int finish_query(ptg_graph *g);
 // Batched queries (batch.hip): `out` is a device buffer of n x V words in
 // the caller's ids; batch_staging gives a library-owned one of 64 x V.
 int run_batch_query(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, uint32_t *out, ptg_stats *st);

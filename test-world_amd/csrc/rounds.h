@@ -35,6 +35,8 @@
 // always expanded, so every round makes progress.
 #pragma once
 
+#include <initializer_list>
+
 #include "device_util.h"
 
 namespace ptg {
@@ -454,7 +456,9 @@ RoundEnv make_env(ptg_graph *g, uint32_t r);
 // Same over another CSR with the graph's vertex set (SSSP's light edges).
 RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint32_t *col);
 // Clears the frontier bitmaps, byte flags and counter ring before a search.
-int reset_rounds(ptg_graph *g);
+// Clears the frontier bitmaps, flags and counter ring, and the caller's
+// `extra` spans (the query's value array and the like), in one launch.
+int reset_rounds(ptg_graph *g, std::initializer_list<FillSpan> extra = {});
 // Seeds round 0 with one source of value src_value (queue mode).
 int launch_seed(ptg_graph *g, uint32_t src, uint32_t src_value, uint32_t *val, uint32_t thr);
 int launch_seed_csr(ptg_graph *g, const uint32_t *off, const uint32_t *col, uint32_t src, uint32_t src_value,

@@ -228,13 +228,16 @@ __global__ void k_seed(RoundEnv e, uint32_t src, uint32_t src_value, uint32_t *v
     }
 }
 
-int reset_rounds(ptg_graph *g) {
-    int rc;
-    for (int i = 0; i < 3; i++)
-        if ((rc = fill_u32(g, g->d_bits[i], g->nwords, 0u))) return rc;
+int reset_rounds(ptg_graph *g, std::initializer_list<FillSpan> extra) {
+    FillSpan s[kMaxSpans];
+    uint32_t k = 0;
+    for (int i = 0; i < 3; i++) s[k++] = FillSpan{g->d_bits[i], g->nwords, 0u};
     for (int i = 0; i < 2; i++)
-        if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_flags[i]), (uint64_t)g->npairs * 16, 0u))) return rc;
-    return fill_u32(g, reinterpret_cast<uint32_t *>(g->d_ring), sizeof(RoundCtr) / 4 * kRing, 0u);
+        s[k++] = FillSpan{reinterpret_cast<uint32_t *>(g->d_flags[i]), (uint64_t)g->npairs * 16, 0u};
+    s[k++] = FillSpan{reinterpret_cast<uint32_t *>(g->d_ring), sizeof(RoundCtr) / 4 * kRing, 0u};
+    if (k + extra.size() > kMaxSpans) return fail(PTG_ERR_INVALID_ARG, "reset_rounds: too many spans");
+    for (const FillSpan &x : extra) s[k++] = x;
+    return fill_spans(g, s, k);
 }
 
 int launch_seed_csr(ptg_graph *g, const uint32_t *off, const uint32_t *col, uint32_t src, uint32_t src_value,

@@ -218,6 +218,7 @@ static void destroy(ptg_graph *g) {
     gfree(g, g->d_part);
     gfree(g, g->d_lpart);
     if (g->h_ring) (void)hipHostFree(g->h_ring);
+    if (g->h_acc) (void)hipHostFree(g->h_acc);
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
     for (auto e : g->tev) (void)hipEventDestroy(e);
@@ -354,7 +355,9 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     }
     TRY(galloc(g, (void **)&g->d_ring, sizeof(RoundCtr) * kRing));
     TRY(galloc(g, (void **)&g->d_acc, sizeof(unsigned long long) * 2 * kShards));
-    if (hipHostMalloc((void **)&g->h_ring, sizeof(RoundCtr) * kRing, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&g->h_ring, sizeof(RoundCtr) * kRing, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&g->h_acc, sizeof(unsigned long long) * 2 * kShards, hipHostMallocDefault) !=
+            hipSuccess)
         TRY(fail(PTG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters)"));
     {
         int sym = 0;
@@ -378,22 +381,29 @@ static int with_output(ptg_graph *g, T *user, F fn) {
     if (!user) return fail(PTG_ERR_INVALID_ARG, "null output pointer");
     const bool dev = is_device_pointer(user);
     uint32_t *target = reinterpret_cast<uint32_t *>(user);
-    int rc;
-    if (!g->relabeled) {
-        if (dev) return fn(reinterpret_cast<T *>(target));
-        if ((rc = fn(reinterpret_cast<T *>(g->d_val)))) return rc;
-        target = g->d_val;
-    } else {
-        if ((rc = fn(reinterpret_cast<T *>(g->d_val)))) return rc;
-        uint32_t *dst = dev ? target : g->d_val2;
-        if ((rc = unpermute(g, g->d_val, dst))) return rc;
-        if (dev) {
-            PTG_HIP(hipStreamSynchronize(g->stream));
-            return PTG_OK;
+    if (!g->relabeled && dev) return fn(reinterpret_cast<T *>(target));
+    // computed into d_val, then unpermuted into the caller's ids and/or copied
+    // to the caller's host buffer
+    auto deliver = [g, user, dev, target]() -> int {
+        const uint32_t *src = g->d_val;
+        if (g->relabeled) {
+            uint32_t *dst = dev ? target : g->d_val2;
+            int rc = unpermute(g, g->d_val, dst);
+            if (rc || dev) return rc;
+            src = dst;
         }
-        target = g->d_val2;
-    }
-    PTG_HIP(hipMemcpyAsync(user, target, (uint64_t)g->V * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+        PTG_HIP(hipMemcpyAsync(user, src, (uint64_t)g->V * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+        return PTG_OK;
+    };
+    g->deliver = deliver;
+    g->delivered = false;
+    int rc = fn(reinterpret_cast<T *>(g->d_val));
+    const bool done = g->delivered;
+    g->deliver = nullptr;
+    g->delivered = false;
+    if (rc) return rc;
+    if (done) return PTG_OK;  // the query enqueued it and waited
+    if ((rc = deliver())) return rc;
     PTG_HIP(hipStreamSynchronize(g->stream));
     return PTG_OK;
 }

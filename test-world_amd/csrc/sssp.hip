@@ -438,9 +438,11 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     const uint32_t *off = split ? g->light.off : g->out.off;
     const uint32_t *col = split ? g->light.col : g->out.col;
     PTG_HIP(hipEventRecord(g->ev0, g->stream));
-    if ((rc = fill_u32(g, dist, g->V, g->cfg.sssp_sentinel)) ||
-        (split && (rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_far), (uint64_t)g->npairs * 16, 0u))) ||
-        (rc = reset_rounds(g)))
+    const FillSpan none{nullptr, 0, 0u};
+    if ((rc = reset_rounds(g, {FillSpan{dist, g->V, g->cfg.sssp_sentinel},
+                               split ? FillSpan{reinterpret_cast<uint32_t *>(g->d_far), (uint64_t)g->npairs * 16, 0u}
+                                     : none,
+                               st ? reach_reset_span(g) : none})))
         return rc;
     // round 0 expands the source (over the light CSR when split)
     const uint32_t thr0 = delta > 0xFFFFFFFEu ? 0xFFFFFFFFu : 1u + delta;
@@ -455,7 +457,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
         return rc;
     }
     PTG_HIP(hipEventRecord(g->ev1, g->stream));
-    PTG_HIP(hipEventSynchronize(g->ev1));
+    if ((st && (rc = reach_stats_launch(g, dist, g->cfg.sssp_sentinel))) || (rc = finish_query(g))) return rc;
     if (st) {
         memset(st, 0, sizeof *st);
         float ms = 0;
@@ -464,7 +466,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
         st->iterations = iters;
         st->frontier_sum = fsum;
         if ((rc = timer.collect(st))) return rc;
-        if ((rc = reach_stats(g, dist, g->cfg.sssp_sentinel, &st->reached, &st->edges_traversed))) return rc;
+        reach_stats_read(g, &st->reached, &st->edges_traversed);
     }
     return PTG_OK;
 }

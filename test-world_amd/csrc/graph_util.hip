@@ -253,12 +253,18 @@ int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t 
     return PTG_OK;
 }
 
-int finish_query(ptg_graph *g) {
+int deliver_output(ptg_graph *g) {
     if (g->deliver) {
         int rc = g->deliver();
         if (rc) return rc;
         g->delivered = true;
     }
+    return PTG_OK;
+}
+
+int finish_query(ptg_graph *g) {
+    int rc = deliver_output(g);
+    if (rc) return rc;
     PTG_HIP(hipStreamSynchronize(g->stream));
     return PTG_OK;
 }

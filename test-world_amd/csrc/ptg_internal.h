@@ -250,7 +250,9 @@ void reach_stats_read(const ptg_graph *g, uint32_t *reached, uint64_t *edges);
 // Fills up to kMaxSpans word ranges with one launch.
 constexpr uint32_t kMaxSpans = 10;
 int fill_spans(ptg_graph *g, const FillSpan *spans, uint32_t k);
-// Enqueues the registered output delivery, if any, then waits for the stream.
+// Enqueues the registered output delivery, if any (it may be enqueued again
+// later: the last one wins); finish_query then also waits for the stream.
+int deliver_output(ptg_graph *g);
 int finish_query(ptg_graph *g);
 // Batched queries (batch.hip): `out` is a device buffer of n x V words in
 // the caller's ids; batch_staging gives a library-owned one of 64 x V.

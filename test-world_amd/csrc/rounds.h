@@ -487,20 +487,32 @@ inline HostSummary host_summary(const RoundCtr &c) {
 // entries 0..r-1 are complete and entry r holds only its header and `any`
 // flag, so idleness is judged on every entry but counts on complete ones.
 // Returns the number of rounds that did work and the sums of their counts.
-template <class Launch, class Idle>
+// `finish` enqueues the query's closing work (end event, statistics, output
+// delivery). With a hint it goes out behind the first batch, before that
+// batch's readback, so a query whose depth repeats waits once; *finished
+// says whether that batch was the last (otherwise the caller enqueues it).
+template <class Launch, class Idle, class Finish>
 int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_t *rounds_out, uint64_t *found_sum,
-                 uint64_t *edges_sum) {
+                 uint64_t *edges_sum, Finish finish, bool *finished) {
     // The first batch covers the previous query's rounds + 1 (one readback when
     // the depth repeats); later batches start at 4 and double up to 16. Rounds
     // launched past the end find an idle entry and return at once.
     uint32_t r = 0, batch = hint ? std::min(hint + 1, 64u) : 2u, rounds = 0;
     const bool hinted = hint != 0;
+    bool first = true;
+    *finished = false;
     uint64_t fs = 0, es = 0;
     uint32_t checked = 0;  // ring entries [0, checked) already summed
     g->frontiers.clear();
     for (;;) {
         for (uint32_t k = 0; k < batch; k++, r++) {
             int rc = launch(r);
+            if (rc) return rc;
+        }
+        const bool spec = hinted && first;
+        first = false;
+        if (spec) {
+            int rc = finish();
             if (rc) return rc;
         }
         int rc = read_ring(g, checked, r + 1 - checked);
@@ -510,6 +522,7 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_
             const RoundCtr &c = g->h_ring[checked % kRing];
             if (idle(c)) {
                 done = true;
+                *finished = spec;
                 break;
             }
             if (checked == r) break;  // not prepared yet: summed after the next batch

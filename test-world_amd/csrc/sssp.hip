@@ -245,9 +245,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxPara
     add_tallies(out, acc);
 }
 
-template <bool kWeighted>
+template <bool kWeighted, class Finish>
 static int run_relax_rounds(ptg_graph *g, RelaxParams p, const uint32_t *off, const uint32_t *col, LaunchTimer &timer,
-                            uint32_t *iters, uint64_t *fsum) {
+                            uint32_t *iters, uint64_t *fsum, Finish finish, bool *finished) {
     return drive_rounds(
         g,
         [&](uint32_t r) -> int {
@@ -265,7 +265,7 @@ static int run_relax_rounds(ptg_graph *g, RelaxParams p, const uint32_t *off, co
             return PTG_OK;
         },
         [](const RoundCtr &c) { return c.any == 0 && c.far_any == 0 && c.need_heavy == 0; },
-        g->last_rounds[kWeighted ? 1 : 2], iters, fsum, nullptr);
+        g->last_rounds[kWeighted ? 1 : 2], iters, fsum, nullptr, finish, finished);
 }
 
 // Counts the light edges (weight < delta) of every vertex: one wave each.
@@ -444,6 +444,22 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
                                      : none,
                                st ? reach_reset_span(g) : none})))
         return rc;
+    // closing work: end event, reach statistics (accumulators cleared again
+    // if an earlier enqueue turned out early), output delivery
+    bool acc_clean = true;
+    auto finish = [&]() -> int {
+        PTG_HIP(hipEventRecord(g->ev1, g->stream));
+        if (st) {
+            const FillSpan z = reach_reset_span(g);
+            int rc2;
+            if ((!acc_clean && (rc2 = fill_spans(g, &z, 1))) ||
+                (rc2 = reach_stats_launch(g, dist, g->cfg.sssp_sentinel)))
+                return rc2;
+            acc_clean = false;
+        }
+        return deliver_output(g);
+    };
+    bool finished = false;
     // round 0 expands the source (over the light CSR when split)
     const uint32_t thr0 = delta > 0xFFFFFFFEu ? 0xFFFFFFFFu : 1u + delta;
     if ((rc = launch_seed_csr(g, off, col, src, 1u, dist, thr0))) return rc;
@@ -452,12 +468,12 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     RelaxParams p{dist, split ? g->d_wl : g->d_w, g->d_far, delta, split ? 1u : 0u,
                   split ? g->heavy.off : nullptr, split ? g->heavy.col : nullptr, split ? g->d_wh : nullptr,
                   split ? g->heavy.bins.q.with(g->heavy.bins.d_ctr) : Frontier{}, defer_deg, window};
-    if ((rc = run_relax_rounds<true>(g, p, off, col, timer, &iters, &fsum))) {
+    if ((rc = run_relax_rounds<true>(g, p, off, col, timer, &iters, &fsum, finish, &finished))) {
         g->defer_dirty = true;
         return rc;
     }
-    PTG_HIP(hipEventRecord(g->ev1, g->stream));
-    if ((st && (rc = reach_stats_launch(g, dist, g->cfg.sssp_sentinel))) || (rc = finish_query(g))) return rc;
+    if (!finished && (rc = finish())) return rc;
+    PTG_HIP(hipStreamSynchronize(g->stream));
     if (st) {
         memset(st, 0, sizeof *st);
         float ms = 0;
@@ -501,7 +517,9 @@ int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uin
     k_seed_all<<<g->grid, kBlock, 0, g->stream>>>(make_env(g, 0), g->V - g->out.bins.n_zero, g->E);
     PTG_HIP(hipGetLastError());
     RelaxParams p{label, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}, 0xFFFFFFFFu, 0u};
-    return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum);
+    bool finished = false;
+    return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum, []() { return PTG_OK; },
+                                   &finished);
 }
 
 }  // namespace ptg

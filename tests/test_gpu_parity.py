@@ -384,3 +384,43 @@ def test_frontier_sizes_match_levels(rmat16):
         assert fr.size == st.iterations == hist.size
         np.testing.assert_array_equal(fr, hist)
         assert int(fr.sum()) == st.frontier_sum == st.reached
+
+
+@pytest.mark.parametrize("flags", [0, pg.FLAG_NO_RELABEL], ids=["relabel", "no_relabel"])
+def test_depth_changes_between_queries(flags):
+    """A query's first launch batch and its closing work (end event, reach
+    statistics, output delivery) are sized from the previous query's depth;
+    shallow and deep queries in turn, to host and device outputs, must match
+    the oracle with exact statistics."""
+    rng = np.random.default_rng(11)
+    src, dst = list(range(299)), list(range(1, 300))  # path 0 -> ... -> 299
+    for leaf in range(301, 400):  # star around 300, both directions
+        src += [300, leaf]
+        dst += [leaf, 300]
+    order = np.argsort(np.array(src), kind="stable")
+    s_arr, col = np.array(src)[order], np.array(dst, dtype=np.uint32)[order]
+    V, E = 400, col.size
+    ptr = np.searchsorted(s_arr, np.arange(V)).astype(np.uint32)
+    w = rng.integers(1, 50, E, dtype=np.uint32)
+    deg = ptr_degrees(ptr, E)
+    buf = pg.DeviceBuffer(4 * V)
+    with pg.Graph(ptr, col, w, config=pg.default_config(flags=flags)) as g:
+        for k, s in enumerate([300, 0, 300, 150, 399, 0, 0, 300]):
+            say("case depth change flags=%d query %d src %d" % (flags, k, s))
+            want_b, n_lev, n_reached, n_edges = oracle.bfs(ptr, col, s)
+            want_d = oracle.dijkstra(ptr, col, w, s)
+            if k % 2:
+                _, st_b = g.bfs(s, out=buf)
+                lv = buf.to_host(np.uint32, V)
+                _, st_d = g.sssp(s, out=buf)
+                d = buf.to_host(np.uint32, V)
+            else:
+                lv, st_b = g.bfs(s)
+                d, st_d = g.sssp(s)
+            np.testing.assert_array_equal(lv, want_b, err_msg="bfs src %d" % s)
+            np.testing.assert_array_equal(d, want_d, err_msg="sssp src %d" % s)
+            assert (st_b.reached, st_b.iterations) == (n_reached, n_lev)
+            assert st_b.edges_traversed == n_edges
+            r_ss = want_d != oracle.SSSP_SENTINEL_SHARE
+            assert st_d.reached == int(r_ss.sum())
+            assert st_d.edges_traversed == int(deg[r_ss].sum())

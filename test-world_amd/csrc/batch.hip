@@ -372,8 +372,36 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_output(BatchEnv e, uint32_t le
 // SSSP
 // ---------------------------------------------------------------------------
 
-// Relaxes the edges [eb, ee) of vertex v for every source whose bit is in
-// `a` (lane k = source k; dv = this lane's distance of v). Rows are read in
+// A chunk of the static chunk list as the round expands it.
+// The wave-uniform fields are moved to scalar registers.
+struct ChunkHead {
+    unsigned long long a;  // sources active at the chunk's vertex
+    uint32_t eb, ee;       // its edges
+    uint32_t dv;           // this lane's distance of the vertex
+    uint32_t mw, mwt;      // columns and weights of the first 64 edges
+};
+
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+
+// Issues the loads a chunk needs before its rows (indexes clamped into the
+// chunk, so every lane issues every load). v, eb, ee and a are wave-uniform.
+__device__ __forceinline__ ChunkHead chunk_head(const BatchEnv &e, uint32_t v, uint32_t eb, uint32_t ee,
+                                                unsigned long long a) {
+    const uint32_t lane = lane_id();
+    ChunkHead h;
+    h.a = ((unsigned long long)uniform((uint32_t)(a >> 32)) << 32) | uniform((uint32_t)a);
+    h.eb = uniform(eb);
+    h.ee = uniform(ee);
+    v = uniform(v);
+    h.dv = e.dist[(uint64_t)v * 64 + lane];
+    const uint32_t i = min(h.eb + lane, h.ee - 1);
+    h.mw = e.col[i];
+    h.mwt = e.wt[i];
+    return h;
+}
+
+// Relaxes the chunk's edges [eb, ee) for every source whose bit is in `a`
+// (lane k = source k; dv = this lane's distance of the chunk's vertex). Rows are read in
 // groups of kP edges, the next group's loads issued before the current
 // group's atomics. Every lane issues every load (lanes of sources not in `a`
 // read the first active source's word of the same row, so no line is added),
@@ -383,17 +411,21 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_output(BatchEnv e, uint32_t le
 // just read (values only fall, so every thread that lowers dist[w][k] sees
 // nd < old and sets the bit; a bit set by a thread another one beat only
 // re-expands w with its current value).
-__device__ __forceinline__ void relax_row(const BatchEnv &e, unsigned long long a, uint32_t dv, uint64_t eb,
-                                          uint64_t ee, uint32_t thr, unsigned long long *nxt, uint64_t &t_nf,
-                                          uint64_t &t_far) {
+__device__ __forceinline__ void relax_chunk(const BatchEnv &e, const ChunkHead &h, uint32_t thr,
+                                            unsigned long long *nxt, uint64_t &t_nf, uint64_t &t_far) {
     constexpr uint32_t kP = 4;  // 8 keeps more loads in flight but spills at 64 VGPRs
     const uint32_t lane = lane_id();
-    const bool act = (a >> lane) & 1ull;
-    const uint32_t sl = act ? lane : (uint32_t)__builtin_ctzll(a);
-    for (uint64_t base = eb; base < ee; base += 64) {
-        const uint32_t cnt = (uint32_t)min((uint64_t)64, ee - base);
-        const uint32_t mw = lane < cnt ? e.col[base + lane] : 0u;
-        const uint32_t mwt = lane < cnt ? e.wt[base + lane] : 0u;
+    const bool act = (h.a >> lane) & 1ull;
+    const uint32_t sl = act ? lane : (uint32_t)__builtin_ctzll(h.a);
+    const uint32_t dv = h.dv;
+    uint32_t mw = h.mw, mwt = h.mwt;
+    for (uint32_t base = h.eb; base < h.ee; base += 64) {
+        const uint32_t cnt = min(64u, h.ee - base);
+        if (base != h.eb) {
+            const uint32_t i = min(base + lane, h.ee - 1);
+            mw = e.col[i];
+            mwt = e.wt[i];
+        }
         // row words of edges q .. q + kP - 1 (indexes past cnt - 1 repeat the last edge)
         auto load = [&](uint32_t q, uint32_t (&old)[kP]) {
 #pragma unroll
@@ -480,17 +512,26 @@ __global__ __launch_bounds__(kBlock, 8) void k_mssp_round(BatchEnv e, uint32_t j
                 cb = e.ch_b[c];
                 a = cur[v];
             }
+            // chunk ends for the 64 chunks at once; each active chunk's head
+            // loads go out while the one before it relaxes
+            uint32_t ce = 0;
+            if (a) ce = (uint32_t)min((uint64_t)e.off[v + 1], (uint64_t)cb + e.chunk);
             uint64_t mm = ballot(a != 0);
-            while (mm) {
+            if (!mm) continue;
+            auto head = [&]() {
                 const uint32_t q = lowest_lane(mm);
                 mm &= mm - 1;
-                const uint32_t vq = read_lane(v, q);
-                const uint32_t bq = read_lane(cb, q);
-                const unsigned long long aq = read_lane64(a, q);
-                const uint64_t eq = min((uint64_t)e.off[vq + 1], (uint64_t)bq + e.chunk);
-                const uint32_t dv = e.dist[(uint64_t)vq * 64 + lane];
-                relax_row(e, aq, dv, bq, eq, thr, nxt, t_nf, t_far);
-                if (lane == 0) t_vis += eq - bq;
+                return chunk_head(e, read_lane(v, q), read_lane(cb, q), read_lane(ce, q), read_lane64(a, q));
+            };
+            ChunkHead ha = head();
+            for (;;) {
+                const bool more = mm != 0;
+                ChunkHead hb = ha;
+                if (more) hb = head();
+                relax_chunk(e, ha, thr, nxt, t_nf, t_far);
+                if (lane == 0) t_vis += ha.ee - ha.eb;
+                if (!more) break;
+                ha = hb;
             }
         }
     } else {

@@ -179,9 +179,9 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
                       g->cfg.bu_to_td > 0 ? g->cfg.bu_to_td : 24.0f, sym};
     uint32_t iters = 0;
     uint64_t reached = 0, edges = 0;
-    auto finish = [&]() -> int {
+    auto finish = [&](bool speculative) -> int {
         PTG_HIP(hipEventRecord(g->ev1, g->stream));
-        return deliver_output(g);
+        return deliver_output(g, speculative);
     };
     bool finished = false;
     rc = drive_rounds(
@@ -198,7 +198,7 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
         [](const RoundCtr &c) { return c.any == 0; }, g->last_rounds[0], &iters, &reached, &edges, finish,
         &finished);
     if (rc) return rc;
-    if (!finished && (rc = finish())) return rc;
+    if (!finished && (rc = finish(false))) return rc;
     PTG_HIP(hipStreamSynchronize(g->stream));
     if (st) {
         memset(st, 0, sizeof *st);

@@ -253,8 +253,8 @@ int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t 
     return PTG_OK;
 }
 
-int deliver_output(ptg_graph *g) {
-    if (g->deliver) {
+int deliver_output(ptg_graph *g, bool speculative) {
+    if (g->deliver && (!speculative || g->deliver_on_device)) {
         int rc = g->deliver();
         if (rc) return rc;
         g->delivered = true;

@@ -158,6 +158,7 @@ struct ptg_graph {
     // caller's buffer), registered by the entry point; a query that ends in
     // finish_query() enqueues it before its one final wait.
     std::function<int()> deliver;
+    bool deliver_on_device = false;  // device-to-device only (cheap to repeat)
     bool delivered = false;
 
     // PageRank state.
@@ -251,8 +252,10 @@ void reach_stats_read(const ptg_graph *g, uint32_t *reached, uint64_t *edges);
 constexpr uint32_t kMaxSpans = 10;
 int fill_spans(ptg_graph *g, const FillSpan *spans, uint32_t k);
 // Enqueues the registered output delivery, if any (it may be enqueued again
-// later: the last one wins); finish_query then also waits for the stream.
-int deliver_output(ptg_graph *g);
+// later: the last one wins); finish_query then also waits for the stream. A
+// speculative call (the query may need more rounds) delivers only to device
+// memory: a copy to the host is made once, after the last round.
+int deliver_output(ptg_graph *g, bool speculative = false);
 int finish_query(ptg_graph *g);
 // Batched queries (batch.hip): `out` is a device buffer of n x V words in
 // the caller's ids; batch_staging gives a library-owned one of 64 x V.

@@ -487,10 +487,11 @@ inline HostSummary host_summary(const RoundCtr &c) {
 // entries 0..r-1 are complete and entry r holds only its header and `any`
 // flag, so idleness is judged on every entry but counts on complete ones.
 // Returns the number of rounds that did work and the sums of their counts.
-// `finish` enqueues the query's closing work (end event, statistics, output
-// delivery). With a hint it goes out behind the first batch, before that
-// batch's readback, so a query whose depth repeats waits once; *finished
-// says whether that batch was the last (otherwise the caller enqueues it).
+// `finish(speculative)` enqueues the query's closing work (end event,
+// statistics, output delivery). With a hint it goes out behind the first
+// batch, before that batch's readback, so a query whose depth repeats waits
+// once; *finished says whether that batch was the last (otherwise the caller
+// enqueues it with speculative = false).
 template <class Launch, class Idle, class Finish>
 int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_t *rounds_out, uint64_t *found_sum,
                  uint64_t *edges_sum, Finish finish, bool *finished) {
@@ -512,7 +513,7 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_
         const bool spec = hinted && first;
         first = false;
         if (spec) {
-            int rc = finish();
+            int rc = finish(true);
             if (rc) return rc;
         }
         int rc = read_ring(g, checked, r + 1 - checked);

@@ -396,6 +396,7 @@ static int with_output(ptg_graph *g, T *user, F fn) {
         return PTG_OK;
     };
     g->deliver = deliver;
+    g->deliver_on_device = dev;
     g->delivered = false;
     int rc = fn(reinterpret_cast<T *>(g->d_val));
     const bool done = g->delivered;

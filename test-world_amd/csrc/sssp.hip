@@ -447,7 +447,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     // closing work: end event, reach statistics (accumulators cleared again
     // if an earlier enqueue turned out early), output delivery
     bool acc_clean = true;
-    auto finish = [&]() -> int {
+    auto finish = [&](bool speculative) -> int {
         PTG_HIP(hipEventRecord(g->ev1, g->stream));
         if (st) {
             const FillSpan z = reach_reset_span(g);
@@ -457,7 +457,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
                 return rc2;
             acc_clean = false;
         }
-        return deliver_output(g);
+        return deliver_output(g, speculative);
     };
     bool finished = false;
     // round 0 expands the source (over the light CSR when split)
@@ -472,7 +472,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
         g->defer_dirty = true;
         return rc;
     }
-    if (!finished && (rc = finish())) return rc;
+    if (!finished && (rc = finish(false))) return rc;
     PTG_HIP(hipStreamSynchronize(g->stream));
     if (st) {
         memset(st, 0, sizeof *st);
@@ -518,7 +518,7 @@ int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uin
     PTG_HIP(hipGetLastError());
     RelaxParams p{label, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}, 0xFFFFFFFFu, 0u};
     bool finished = false;
-    return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum, []() { return PTG_OK; },
+    return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum, [](bool) { return PTG_OK; },
                                    &finished);
 }
 

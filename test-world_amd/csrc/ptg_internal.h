@@ -187,6 +187,7 @@ struct ptg_graph {
     // zero between rounds) and the automatic deferral window (0 = not
     // computed yet).
     uint32_t *d_defer = nullptr;
+    uint2 *d_cw = nullptr;  // (column, weight) pairs for the default SSSP rounds, built on first use
     uint32_t auto_window = 0;
     bool defer_dirty = false;  // the deferred bitmap may hold bits of a failed query
 

@@ -208,7 +208,7 @@ __device__ __forceinline__ void add_tallies(RoundCtr *out, const Acc &acc) {
         if (tf) atomicAdd(&out->found_sh[shard], tf);
         if (tfa) atomicAdd(&out->far_sh[shard], tfa);
         if (te) atomicAdd(&out->edges_sh[shard], te);
-        if (tm > out->fmin_inv) atomicMax(&out->fmin_inv, tm);
+        if (tm && tm > out->fmin_inv) atomicMax(&out->fmin_inv, tm);  // no read when there is nothing to add
     }
 }
 

@@ -44,12 +44,14 @@ constexpr uint32_t kDefaultDeferDegree = 64;
 // current value, which changes no result. Queue mode dedupes near vertices
 // through the output bitmap so each is appended once; flags mode (large
 // frontiers) stores a byte per improved vertex. Far vertices are marked in a
-// byte array in both modes.
-template <bool kWeighted>
+// byte array in both modes. kPacked reads each edge's column and weight as
+// one 8-byte pair (cw) instead of two 4-byte loads.
+template <bool kWeighted, bool kPacked = false>
 struct Relax {
     const uint32_t *__restrict__ off;
     const uint32_t *__restrict__ col;
     const uint32_t *__restrict__ wt;
+    const uint2 *__restrict__ cw;
     uint32_t *val;
     uint32_t *fout;
     uint8_t *flout;
@@ -88,8 +90,14 @@ struct Relax {
     __device__ __forceinline__ Pre load(bool valid, uint32_t sv, uint64_t e) const {
         Pre p;
         p.valid = valid;
-        p.w = valid ? col[e] : 0u;
-        p.nv = kWeighted ? sv + (valid ? wt[e] : 0u) : sv;
+        if constexpr (kPacked) {
+            const uint2 x = valid ? cw[e] : make_uint2(0u, 0u);
+            p.w = x.x;
+            p.nv = sv + x.y;
+        } else {
+            p.w = valid ? col[e] : 0u;
+            p.nv = kWeighted ? sv + (valid ? wt[e] : 0u) : sv;
+        }
         p.cur = valid ? val[p.w] : 0u;
         return p;
     }
@@ -157,6 +165,7 @@ __device__ __forceinline__ void sssp_split(const RoundEnv &e, const uint32_t *va
 struct RelaxParams {
     uint32_t *val;
     const uint32_t *wt;  // weights of the CSR the rounds expand (light edges)
+    const uint2 *cw;     // (column, weight) pairs of the full CSR (one unbounded bucket)
     uint8_t *far;
     uint32_t delta;
     uint32_t split;      // 1: heavy edges live in the CSR below and get a round per bucket
@@ -199,8 +208,9 @@ struct HeavyRelax {
 // The far set is non-empty after the round when it was before an expand
 // round, when anything was added or kept, and (conservatively) after a heavy
 // round. kBuckets = false (one unbounded bucket, no split: the far set stays
-// empty) compiles the expand round only, with fewer registers. Eight waves
-// per SIMD keep the whole persistent grid (8 workgroups per CU) resident.
+// empty) compiles the expand round only, with fewer registers, and reads
+// packed (column, weight) pairs. Eight waves per SIMD keep the whole
+// persistent grid (8 workgroups per CU) resident.
 template <bool kWeighted, bool kBuckets>
 __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxParams p) {
     RoundCtr *in = e.ring + e.r % kRing;
@@ -231,7 +241,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxPara
     Acc acc;
     if (expand) {
         const uint32_t lim = p.defer_deg != 0xFFFFFFFFu ? defer_limit(in, p.window) : 0xFFFFFFFFu;
-        Relax<kWeighted> op{e.off, e.col, p.wt, p.val, e.fout, e.flout, p.far, thr, p.defer_deg, lim};
+        Relax<kWeighted, kWeighted && !kBuckets> op{e.off,   e.col,   p.wt, p.cw,  p.val,       e.fout,
+                                                     e.flout, p.far,   thr,  p.defer_deg, lim};
         expand_frontier(e, in, op, em, acc);
         if (lim != 0xFFFFFFFFu) requeue_deferred(e, in, op, em, acc);
     } else if constexpr (kBuckets) {
@@ -378,12 +389,15 @@ void release_sssp_split(ptg_graph *g) {
     free_split(g);
     gfree(g, g->d_defer);
     g->d_defer = nullptr;
+    gfree(g, g->d_cw);
+    g->d_cw = nullptr;
 }
 
 uint64_t sssp_split_need(const ptg_graph *g) {
     if (!g->weighted) return 0;
     uint64_t need = g->d_defer ? 0 : 4ull * g->nwords;
     const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kUnbounded;
+    if (delta == kUnbounded && !g->d_cw) need += 8ull * std::max<uint64_t>(g->E, 1);
     if (delta != kUnbounded && !(g->split_delta == delta && g->light.off))
         need += 8ull * g->E + 8ull * (g->V + 1) + 4ull * g->V + (64u << 10);
     return need;
@@ -396,6 +410,25 @@ __global__ __launch_bounds__(kBlock) void k_weight_sum(const uint32_t *__restric
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < E; i += (uint64_t)gridDim.x * kBlock) sum += w[i];
     sum = wave_sum_u64(sum);
     if (lane_id() == 0) atomicAdd(&acc[(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kShards], sum);
+}
+
+// (column, weight) of every edge side by side, for the default rounds.
+__global__ __launch_bounds__(kBlock) void k_pack_edges(const uint32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ w, uint64_t E,
+                                                       uint2 *__restrict__ cw) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < E; i += (uint64_t)gridDim.x * kBlock)
+        cw[i] = make_uint2(col[i], w[i]);
+}
+
+static int ensure_packed(ptg_graph *g) {
+    if (g->d_cw) return PTG_OK;
+    int rc = galloc(g, (void **)&g->d_cw, 8ull * std::max<uint64_t>(g->E, 1));
+    if (rc) return rc;
+    if (g->E) {
+        k_pack_edges<<<g->grid, kBlock, 0, g->stream>>>(g->out.col, g->d_w, g->E, g->d_cw);
+        PTG_HIP(hipGetLastError());
+    }
+    return PTG_OK;
 }
 
 // Allocates the (zeroed) deferred bitmap and, once per graph, sets the
@@ -431,7 +464,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     LaunchTimer timer(g);
     const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kUnbounded;
     const bool split = delta != kUnbounded;
-    if (split && (rc = ensure_split(g, delta))) return rc;
+    if (split ? (rc = ensure_split(g, delta)) : (rc = ensure_packed(g))) return rc;
     if ((rc = ensure_defer(g))) return rc;
     const uint32_t defer_deg = g->cfg.sssp_defer_degree ? g->cfg.sssp_defer_degree : kDefaultDeferDegree;
     const uint32_t window = g->cfg.sssp_defer_window ? g->cfg.sssp_defer_window : g->auto_window;
@@ -465,7 +498,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     if ((rc = launch_seed_csr(g, off, col, src, 1u, dist, thr0))) return rc;
     uint32_t iters = 0;
     uint64_t fsum = 0;
-    RelaxParams p{dist, split ? g->d_wl : g->d_w, g->d_far, delta, split ? 1u : 0u,
+    RelaxParams p{dist, split ? g->d_wl : g->d_w, split ? nullptr : g->d_cw, g->d_far, delta, split ? 1u : 0u,
                   split ? g->heavy.off : nullptr, split ? g->heavy.col : nullptr, split ? g->d_wh : nullptr,
                   split ? g->heavy.bins.q.with(g->heavy.bins.d_ctr) : Frontier{}, defer_deg, window};
     if ((rc = run_relax_rounds<true>(g, p, off, col, timer, &iters, &fsum, finish, &finished))) {
@@ -516,7 +549,7 @@ int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uin
     if ((rc = reset_rounds(g))) return rc;
     k_seed_all<<<g->grid, kBlock, 0, g->stream>>>(make_env(g, 0), g->V - g->out.bins.n_zero, g->E);
     PTG_HIP(hipGetLastError());
-    RelaxParams p{label, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}, 0xFFFFFFFFu, 0u};
+    RelaxParams p{label, nullptr, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}, 0xFFFFFFFFu, 0u};
     bool finished = false;
     return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum, [](bool) { return PTG_OK; },
                                    &finished);

@@ -170,27 +170,26 @@ __global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32
     const uint32_t gw = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     uint64_t cnt = 0, edges = 0;
-    // four 64-vertex tiles per step; values and offsets are loaded together
-    // (lane l's next offset is lane l+1's; lane 63 and vertex V-1 load their own)
+    // four 64-vertex tiles per step. Every lane issues every load (indexes
+    // clamped into range, masked afterwards), so the compiler can keep all of
+    // a step's loads in flight; off[v + 1] hits the lines off[v] brought in.
     constexpr uint32_t kU = 4;
     for (uint64_t base = (uint64_t)gw * 64; base < V; base += (uint64_t)nw * 64 * kU) {
-        uint32_t x[kU], o[kU], o63[kU];
+        uint32_t x[kU], o[kU], o1[kU];
 #pragma unroll
         for (uint32_t j = 0; j < kU; j++) {
             const uint64_t v = base + (uint64_t)j * nw * 64 + lane;
-            x[j] = v < V ? val[v] : unreached;
-            o[j] = v < V ? off[v] : 0u;
-            o63[j] = v < V && (lane == 63 || v + 1 == V) ? off[v + 1] : 0u;
+            const uint64_t vc = v < V ? v : V - 1;
+            x[j] = val[vc];
+            o[j] = off[vc];
+            o1[j] = off[vc + 1];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kU; j++) {
-            const uint32_t up = (uint32_t)__shfl_down((int)o[j], 1, 64);
             const uint64_t v = base + (uint64_t)j * nw * 64 + lane;
-            const uint32_t next = lane == 63 || v + 1 == V ? o63[j] : up;
-            // lanes past V hold unreached, so their (garbage) next is unused
             if (v < V && x[j] != unreached) {
                 cnt++;
-                edges += next - o[j];
+                edges += o1[j] - o[j];
             }
         }
     }

@@ -107,7 +107,7 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     int rc = PTG_OK;
     if (what & PTG_TUNE_BFS) {
         if (!rc) rc = t.knob(kBfs, &c.td_to_bu, std::vector<float>{4, 8, 14, 24, 40});
-        if (!rc) rc = t.knob(kBfs, &c.bu_to_td, std::vector<float>{8, 16, 24, 48});
+        if (!rc) rc = t.knob(kBfs, &c.bu_to_td, std::vector<float>{8, 16, 24, 48, 96, 200});
     }
     if ((what & PTG_TUNE_SSSP) && g->weighted) {
         // 0 = automatic (one unbounded bucket; degree 64; mean weight / 16)

@@ -261,12 +261,6 @@ int deliver_output(ptg_graph *g, bool speculative) {
     return PTG_OK;
 }
 
-int finish_query(ptg_graph *g) {
-    int rc = deliver_output(g);
-    if (rc) return rc;
-    PTG_HIP(hipStreamSynchronize(g->stream));
-    return PTG_OK;
-}
 
 // ---------------------------------------------------------------------------
 // Row checksums: sum over v of row[v] * (2v + 1) mod 2^64, per row.

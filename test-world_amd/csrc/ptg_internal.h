@@ -155,8 +155,9 @@ struct ptg_graph {
     unsigned long long *d_acc = nullptr; // 2 x kShards accumulators for statistics kernels
     unsigned long long *h_acc = nullptr; // pinned mirror of d_acc
     // Output delivery of the running query (unpermute and/or copy to the
-    // caller's buffer), registered by the entry point; a query that ends in
-    // finish_query() enqueues it before its one final wait.
+    // caller's buffer), registered by the entry point. BFS and SSSP enqueue
+    // it themselves (deliver_output) before their final wait; for the other
+    // queries the entry point runs it after the query returns.
     std::function<int()> deliver;
     bool deliver_on_device = false;  // device-to-device only (cheap to repeat)
     bool delivered = false;
@@ -253,11 +254,10 @@ void reach_stats_read(const ptg_graph *g, uint32_t *reached, uint64_t *edges);
 constexpr uint32_t kMaxSpans = 10;
 int fill_spans(ptg_graph *g, const FillSpan *spans, uint32_t k);
 // Enqueues the registered output delivery, if any (it may be enqueued again
-// later: the last one wins); finish_query then also waits for the stream. A
-// speculative call (the query may need more rounds) delivers only to device
-// memory: a copy to the host is made once, after the last round.
+// later: the last one wins). A speculative call (the query may need more
+// rounds) delivers only to device memory: a copy to the host is made once,
+// after the last round.
 int deliver_output(ptg_graph *g, bool speculative = false);
-int finish_query(ptg_graph *g);
 // Batched queries (batch.hip): `out` is a device buffer of n x V words in
 // the caller's ids; batch_staging gives a library-owned one of 64 x V.
 int run_batch_query(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, uint32_t *out, ptg_stats *st);

@@ -412,10 +412,19 @@ class DeviceBuffer:
         n = min(self.nbytes, other.nbytes) if nbytes is None else nbytes
         _check(lib().ptg_memcpy_d2d(self.ptr, other.ptr, n), "ptg_memcpy_d2d")
 
+    def view(self, offset, nbytes):
+        """A window [offset, offset + nbytes) of this buffer that does not own
+        its memory (an output at any 4-byte offset, for example)."""
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("view outside the buffer")
+        v = DeviceBuffer.__new__(DeviceBuffer)
+        v.nbytes, v.ptr, v._owner = nbytes, self.ptr + offset, self
+        return v
+
     def free(self):
-        if self.ptr:
+        if self.ptr and getattr(self, "_owner", None) is None:
             lib().ptg_device_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:

@@ -424,3 +424,32 @@ def test_depth_changes_between_queries(flags):
             r_ss = want_d != oracle.SSSP_SENTINEL_SHARE
             assert st_d.reached == int(r_ss.sum())
             assert st_d.edges_traversed == int(deg[r_ss].sum())
+
+
+@pytest.mark.parametrize("flags", [0, pg.FLAG_NO_RELABEL], ids=["relabel", "no_relabel"])
+def test_device_outputs_at_unaligned_offsets(flags):
+    """Results written to device memory at offsets that are not 16-byte
+    aligned (the value array of a graph without relabeling is the caller's
+    buffer, filled in place) match the host results, and the words around
+    the window keep their contents."""
+    rng = np.random.default_rng(21)
+    V = 1003
+    ptr, col, w = random_directed(rng, V, 6000, 40)
+    pad = 8
+    guard = np.full(V + 2 * pad, 0xA5A5A5A5, dtype=np.uint32)
+    buf = pg.DeviceBuffer(4 * guard.size)
+    with pg.Graph(ptr, col, w, config=pg.default_config(flags=flags)) as g:
+        want = {"bfs": g.bfs(5)[0], "sssp": g.sssp(5)[0], "cc": g.cc()[0]}
+        for off in (1, 2, 3, 5):
+            for kind in ("bfs", "sssp", "cc"):
+                say("case unaligned output flags=%d offset %d %s" % (flags, off, kind))
+                buf.from_host(guard)
+                view = buf.view(4 * (pad + off), 4 * V)
+                if kind == "cc":
+                    g.cc(out=view)
+                else:
+                    getattr(g, kind)(5, out=view)
+                got = buf.to_host(np.uint32, guard.size)
+                np.testing.assert_array_equal(got[pad + off:pad + off + V], want[kind], err_msg="%s at +%d" % (kind, off))
+                np.testing.assert_array_equal(got[:pad + off], guard[:pad + off])
+                np.testing.assert_array_equal(got[pad + off + V:], guard[pad + off + V:])

@@ -391,7 +391,7 @@ def main():
 
     per_kind = kind_stats(acc)
     per_kind_warm = kind_stats(wacc)
-    kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false>", "cc": "k_expand<Hook>",
+    kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false>", "cc": "k_expand<HookRest>",
                     "pagerank": "k_pr_gather"}
     achieved = per_kind[dom]["achieved_gbs"]
     roofline = None if args.no_timing else {

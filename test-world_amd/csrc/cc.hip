@@ -3,8 +3,16 @@
 // The reference's cc_kernel (ptGraph/gpu_kernels.cu:136-150) propagates
 // value[v] = v along out-edges to a fixed point: label[v] = the smallest id u
 // with a path u -> v. On a symmetric graph that is the smallest id of v's
-// component, which union-find with min-id roots computes in one pass over the
-// edges; on other graphs the frontier label propagation (sssp.hip) runs.
+// component, which union-find with min-id roots computes; on other graphs the
+// frontier label propagation (sssp.hip) runs.
+//
+// The union-find follows Afforest (Sutton, Ben-Nun, Barak, IPDPS 2018): every
+// vertex is first linked to its first kAfforestK neighbours; the component
+// most of kAfforestSamples sampled vertices fall in (the giant component of a
+// power-law graph) is found; then only vertices outside it link their
+// remaining edges. An edge skipped from both ends joins two vertices that
+// were both in that component when they were checked, and components only
+// grow, so no link is lost.
 // Ids are the caller's: with relabeling the union-find keys are the original
 // ids, so every root is its component's minimum original id.
 #include <string.h>
@@ -49,15 +57,81 @@ __device__ __forceinline__ void uf_unite(uint32_t *par, const uint32_t *key, uin
     }
 }
 
-struct Hook {
+constexpr uint32_t kAfforestK = 2;
+constexpr uint32_t kAfforestSamples = 1024;
+
+// Links every vertex to its first kAfforestK neighbours.
+__global__ __launch_bounds__(kBlock) void k_uf_link_first(uint32_t V, const uint32_t *__restrict__ off,
+                                                          const uint32_t *__restrict__ col,
+                                                          const uint32_t *__restrict__ key, uint32_t *par) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        const uint32_t b = off[v], d = min(off[v + 1] - b, kAfforestK);
+        for (uint32_t i = 0; i < d; i++) uf_unite(par, key, v, col[b + i]);
+    }
+}
+
+// One workgroup of kAfforestSamples threads: the root shared by the most
+// sampled vertices (ties to the smaller root) goes to *giant. The samples are
+// a fixed hash of the thread index, so the choice is repeatable.
+__global__ __launch_bounds__(kAfforestSamples) void k_uf_sample(uint32_t V, uint32_t *par, uint32_t *giant) {
+    constexpr uint32_t kSlots = 2 * kAfforestSamples;
+    __shared__ uint32_t s_key[kSlots], s_cnt[kSlots];
+    __shared__ unsigned long long s_best[kAfforestSamples / 64];
+    for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x) {
+        s_key[i] = 0xFFFFFFFFu;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();
+    uint32_t h = threadIdx.x * 0x9E3779B9u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    const uint32_t v = (uint32_t)(((uint64_t)h * V) >> 32);
+    const uint32_t r = uf_root(par, v);
+    uint32_t slot = (r * 0x9E3779B1u) >> 21;  // 11 bits = kSlots
+    for (;;) {
+        const uint32_t old = atomicCAS(&s_key[slot], 0xFFFFFFFFu, r);
+        if (old == 0xFFFFFFFFu || old == r) {
+            atomicAdd(&s_cnt[slot], 1u);
+            break;
+        }
+        slot = (slot + 1) & (kSlots - 1);
+    }
+    __syncthreads();
+    // largest count, then smallest root: maximise (count << 32) | ~root
+    unsigned long long best = 0;
+    for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x)
+        if (s_cnt[i]) best = max(best, ((unsigned long long)s_cnt[i] << 32) | (uint32_t)~s_key[i]);
+    best = wave_max_u64(best);
+    if (lane_id() == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < kAfforestSamples / 64; w++) best = max(best, s_best[w]);
+        *giant = ~(uint32_t)best;
+    }
+}
+
+// The remaining edges (from the (kAfforestK+1)-th on) of vertices outside the
+// giant component; vertex() decides once per vertex (or hub chunk).
+struct HookRest {
+    const uint32_t *__restrict__ off;
     const uint32_t *__restrict__ col;
     const uint32_t *__restrict__ key;
     uint32_t *par;
-    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return v; }
-    __device__ __forceinline__ void edge(bool valid, uint32_t v, uint64_t e) const {
-        if (!valid) return;
-        const uint32_t u = col[e];
-        if (u < v) uf_unite(par, key, v, u);
+    const uint32_t *giant;
+    struct Src {
+        uint32_t v, first;  // first = ~0u: nothing to link
+    };
+    __device__ __forceinline__ Src vertex(uint32_t v) const {
+        const uint32_t b = off[v];
+        if ((uint64_t)b + kAfforestK >= off[v + 1] || uf_root(par, v) == uf_root(par, *giant))
+            return Src{v, 0xFFFFFFFFu};
+        return Src{v, b + kAfforestK};
+    }
+    __device__ __forceinline__ bool active(Src s) const { return s.first != 0xFFFFFFFFu; }
+    __device__ __forceinline__ void edge(bool valid, Src s, uint64_t e) const {
+        if (valid && e >= s.first) uf_unite(par, key, s.v, col[e]);
     }
 };
 
@@ -142,12 +216,16 @@ int run_cc(ptg_graph *g, uint32_t *label, ptg_stats *st) {
     const uint32_t *key = g->d_inv;  // null when the device keeps the caller's ids
     PTG_HIP(hipEventRecord(g->ev0, g->stream));
     if (sym) {
-        k_uf_init<<<g->grid, kBlock, 0, g->stream>>>(g->V, g->out.off, g->out.col, key, label);
+        // every union-find launch is timed: together they are the computation
+        uint32_t *giant = reinterpret_cast<uint32_t *>(g->d_acc);  // count_roots clears it afterwards
+        const uint32_t *off = g->out.off, *col = g->out.col;
+        timer.launch(k_uf_init, dim3(g->grid), dim3(kBlock), g->V, off, col, key, label);
+        timer.launch(k_uf_link_first, dim3(g->grid), dim3(kBlock), g->V, off, col, key, label);
+        if (g->V) timer.launch(k_uf_sample, dim3(1), dim3(kAfforestSamples), g->V, label, giant);
         const Frontier fin = g->out.bins.q.with(g->out.bins.d_ctr);
-        timer.launch(k_expand<Hook>, dim3(g->grid), dim3(kBlock), fin, (const uint32_t *)g->out.off,
-                     Hook{g->out.col, key, label});
-        k_uf_flatten<<<g->grid, kBlock, 0, g->stream>>>(g->V, label);
-        if (key) k_map_keys<<<g->grid, kBlock, 0, g->stream>>>(g->V, key, label);
+        timer.launch(k_expand<HookRest>, dim3(g->grid), dim3(kBlock), fin, off, HookRest{off, col, key, label, giant});
+        timer.launch(k_uf_flatten, dim3(g->grid), dim3(kBlock), g->V, label);
+        if (key) timer.launch(k_map_keys, dim3(g->grid), dim3(kBlock), g->V, key, label);
         PTG_HIP(hipGetLastError());
     } else {
         k_init_keys<<<g->grid, kBlock, 0, g->stream>>>(g->V, key, label);

@@ -39,6 +39,16 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     return x;
 }
 
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, d, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), d, 64);
+        const uint64_t y = (uint64_t)hi << 32 | lo;
+        x = x > y ? x : y;
+    }
+    return x;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     for (int d = 32; d >= 1; d >>= 1) {
         uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, d, 64);

@@ -453,3 +453,66 @@ def test_device_outputs_at_unaligned_offsets(flags):
                 np.testing.assert_array_equal(got[pad + off:pad + off + V], want[kind], err_msg="%s at +%d" % (kind, off))
                 np.testing.assert_array_equal(got[:pad + off], guard[:pad + off])
                 np.testing.assert_array_equal(got[pad + off + V:], guard[pad + off + V:])
+
+
+def symmetric_csr(V, pairs):
+    """Reference-layout CSR (V offsets) of the symmetric closure of `pairs`,
+    without self-loops or duplicates."""
+    a = np.array([p[0] for p in pairs] + [p[1] for p in pairs], dtype=np.int64)
+    b = np.array([p[1] for p in pairs] + [p[0] for p in pairs], dtype=np.int64)
+    keep = a != b
+    e = np.unique(np.stack([a[keep], b[keep]], axis=1), axis=0)
+    ptr = np.searchsorted(e[:, 0], np.arange(V)).astype(np.uint32)
+    return ptr, e[:, 1].astype(np.uint32)
+
+
+def cc_shapes(rng):
+    V = 3000
+    shapes = {}
+    # two dense halves of equal size: no single giant component
+    pairs = [tuple(rng.integers(0, 1500, 2)) for _ in range(9000)]
+    pairs += [tuple(rng.integers(1500, 3000, 2)) for _ in range(9000)]
+    shapes["two_giants"] = pairs
+    # 300 random trees of 10 vertices
+    perm = rng.permutation(V)
+    shapes["many_small"] = [(perm[10 * t + i], perm[10 * t + rng.integers(0, i)]) for t in range(300) for i in range(1, 10)]
+    # a dense cluster on random ids plus chains of 2-5 vertices elsewhere
+    ids = rng.permutation(V)
+    cl, rest = ids[:2000], ids[2000:]
+    pairs = [(cl[rng.integers(0, 2000)], cl[rng.integers(0, 2000)]) for _ in range(12000)]
+    i = 0
+    while i + 1 < rest.size:
+        n = int(rng.integers(2, 6))
+        pairs += [(rest[j], rest[j + 1]) for j in range(i, min(i + n, rest.size) - 1)]
+        i += n
+    shapes["giant_plus_chains"] = pairs
+    # hubs in a ring whose two lowest-id neighbours are leaves of their own
+    # (ids 0-19): without relabeling the hubs meet only through edges past
+    # each one's first two neighbours
+    pairs = []
+    for h in range(10):
+        hub = 100 + 290 * h
+        pairs += [(hub, 2 * h), (hub, 2 * h + 1)]
+        pairs += [(hub, leaf) for leaf in range(hub + 1, hub + 200)]
+        pairs.append((hub, 100 + 290 * ((h + 1) % 10)))
+    shapes["hubs_leaves_first"] = pairs
+    return V, shapes
+
+
+@pytest.mark.parametrize("flags", [0, pg.FLAG_NO_RELABEL], ids=["relabel", "no_relabel"])
+def test_cc_component_shapes(flags):
+    """Union-find CC (first neighbours linked, the most sampled component
+    skipped, the rest linked) against the oracle on symmetric graphs with no
+    giant component, many small ones, a giant one with chains beside it, and
+    hubs joined only through their later edges."""
+    V, shapes = cc_shapes(np.random.default_rng(31))
+    for name, pairs in shapes.items():
+        ptr, col = symmetric_csr(V, pairs)
+        want = oracle.cc_union_find(ptr, col)
+        say("case cc shape %s flags=%d components %d" % (name, flags, len(np.unique(want))))
+        with pg.Graph(ptr, col, config=pg.default_config(flags=flags)) as g:
+            assert g.is_symmetric()
+            for _ in range(2):
+                lab, st = g.cc()
+                np.testing.assert_array_equal(lab, want, err_msg=name)
+                assert st.reached == len(np.unique(want))

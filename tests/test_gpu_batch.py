@@ -217,3 +217,26 @@ def test_autotune_and_set_config():
             with pytest.raises(pg.PtgError) as e:
                 g.tune(**bad)
             assert e.value.code == pg.ERR_INVALID_ARG
+
+
+def test_batch_er16_and_cc():
+    """Batched BFS/SSSP on a uniform random (Erdos-Renyi) graph of scale 16:
+    no hubs, so nearly all work falls in the small-degree bins. Rows equal
+    the single queries; CC equals the oracle's labels."""
+    say("case batch er16")
+    csr = pg.gen_er(16, 16, seed=5).add_weights(1, 64, seed=6)
+    ptr, col, w = csr.node_pointers, csr.cols, csr.weights
+    deg = np.diff(csr.offsets.astype(np.int64))
+    srcs = np.random.default_rng(7).choice(np.nonzero(deg > 0)[0], 64, replace=False).astype(np.uint32)
+    with pg.Graph.from_csr(csr) as g:
+        assert g.is_symmetric()
+        lv, _ = g.bfs_batch(srcs)
+        dist, _ = g.sssp_batch(srcs)
+        for k in range(0, 64, 9):
+            s = int(srcs[k])
+            np.testing.assert_array_equal(lv[k], g.bfs(s)[0], err_msg="bfs row %d" % k)
+            np.testing.assert_array_equal(dist[k], g.sssp(s)[0], err_msg="sssp row %d" % k)
+        check_rows(ptr, col, w, srcs[:2], lv[:2], dist[:2])
+        lab, st = g.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_union_find(ptr, col))
+        assert st.reached == len(np.unique(lab))

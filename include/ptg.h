@@ -84,9 +84,9 @@ typedef struct ptg_config {
                                      light); 0 = one unbounded bucket, no split */
     uint32_t flags;               /* PTG_FLAG_* */
     uint32_t batch_delta;         /* near/far step of batched SSSP; 0 = auto */
-    uint32_t batch_grid;          /* workgroups of the batched kernels; 0 = grid_blocks */
+    uint32_t batch_grid;          /* workgroups of the batched kernels; 0 = 4 x grid_blocks */
     float batch_td_to_bu;         /* batched BFS goes bottom-up when m_f > m_u / this; 0 = td_to_bu */
-    uint32_t batch_chunk;         /* edges per static chunk of batched SSSP; 0 = auto (256) */
+    uint32_t batch_chunk;         /* edges per static chunk of batched SSSP; 0 = auto (128) */
     uint32_t order_hub_degree;    /* vertex order on the device: 0 = by descending degree; d > 0 =
                                      vertices of degree > d by degree, then breadth-first from them */
     uint32_t sssp_defer_degree;   /* SSSP: a vertex of degree > this waits in the frontier while its

@@ -97,9 +97,9 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     if (c.td_to_bu <= 0) c.td_to_bu = 14.0f;
     if (c.bu_to_td <= 0) c.bu_to_td = 24.0f;
     if (c.batch_td_to_bu <= 0) c.batch_td_to_bu = c.td_to_bu;
-    if (!c.batch_grid) c.batch_grid = 2 * g->grid;
+    if (!c.batch_grid) c.batch_grid = 4 * g->grid;
     if (!c.batch_delta) c.batch_delta = 256;
-    if (!c.batch_chunk) c.batch_chunk = 256;
+    if (!c.batch_chunk) c.batch_chunk = 128;
     Tuner t{g, {}, {}};
     for (int i = 0; i < n && i < 4; i++) t.plain.push_back(g->relabeled ? g->h_perm[srcs[i]] : srcs[i]);
     for (int i = 0; i < n && i < 64; i++) t.batch.push_back(srcs[i]);

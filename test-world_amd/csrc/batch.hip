@@ -40,7 +40,7 @@ constexpr uint32_t kBRing = 64;        // counter ring entries (> replay length 
 constexpr uint32_t kBfsReplay = 8;     // BFS levels per graph replay
 constexpr uint32_t kSsspReplay = 16;   // SSSP rounds per graph replay
 constexpr uint32_t kLevelCap = 32;     // BFS levels kept as masks; deeper -> per-source path
-constexpr uint32_t kBatchChunk = 256;  // SSSP: default edges per static chunk
+constexpr uint32_t kBatchChunk = 128;  // SSSP: default edges per static chunk
 
 // Counters of one level / round, 16-way sharded.
 struct BCtr {
@@ -708,7 +708,7 @@ static BatchEnv make_batch_env(ptg_graph *g, int sym) {
     e.alpha = g->cfg.batch_td_to_bu > 0 ? g->cfg.batch_td_to_bu : (g->cfg.td_to_bu > 0 ? g->cfg.td_to_bu : 14.0f);
     e.delta = g->cfg.batch_delta ? g->cfg.batch_delta : 256u;
     e.symmetric = sym;
-    e.grid = g->cfg.batch_grid ? g->cfg.batch_grid : 2 * g->grid;
+    e.grid = g->cfg.batch_grid ? g->cfg.batch_grid : 4 * g->grid;
     return e;
 }
 

@@ -53,7 +53,7 @@ int fill_spans(ptg_graph *g, const FillSpan *spans, uint32_t k) {
             total += spans[i].n;
         }
     if (!f.k) return PTG_OK;
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total / 4 + kBlock - 1) / kBlock + 1, g->grid);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total / 4 + kBlock - 1) / kBlock + 1, std::min(g->grid, 512u));
     k_fill_spans<<<blocks, kBlock, 0, g->stream>>>(f);
     PTG_HIP(hipGetLastError());
     return PTG_OK;
@@ -217,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_reach_stats(uint32_t V, const uint32
 FillSpan reach_reset_span(ptg_graph *g) { return FillSpan{reinterpret_cast<uint32_t *>(g->d_acc), 4 * kShards, 0u}; }
 
 static int reach_count(ptg_graph *g, const uint32_t *val, uint32_t unreached, unsigned long long *h) {
-    k_reach_stats<<<g->grid, kBlock, 0, g->stream>>>(g->V, val, unreached, g->out.off, g->d_acc);
+    // 512 workgroups: twice as fast as the 2048 of a round grid for this scan
+    k_reach_stats<<<std::min(g->grid, 512u), kBlock, 0, g->stream>>>(g->V, val, unreached, g->out.off, g->d_acc);
     PTG_HIP(hipGetLastError());
     PTG_HIP(hipMemcpyAsync(h, g->d_acc, sizeof(unsigned long long) * 2 * kShards, hipMemcpyDeviceToHost, g->stream));
     return PTG_OK;

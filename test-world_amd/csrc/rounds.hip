@@ -254,7 +254,7 @@ int launch_seed(ptg_graph *g, uint32_t src, uint32_t src_value, uint32_t *val, u
 }
 
 int launch_prepare(ptg_graph *g, const RoundEnv &e) {
-    k_prepare<<<g->grid, kBlock, 0, g->stream>>>(e);
+    k_prepare<<<prepare_grid(g->grid), kBlock, 0, g->stream>>>(e);
     PTG_HIP(hipGetLastError());
     return PTG_OK;
 }

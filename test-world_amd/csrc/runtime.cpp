@@ -85,12 +85,13 @@ void gfree(ptg_graph *g, void *p) {
 }
 
 uint32_t chunk_queue_cap(uint32_t V, uint32_t grid) {
-    // The bitmap pre-pass gives workgroup b the tiles [b*T, (b+1)*T) and
-    // appends to shard b % kShards; queue-mode rounds append at most
-    // min(V, kQueueCap) entries to any shard.
+    // The bitmap pre-pass (prepare_grid(grid) workgroups) gives workgroup b
+    // the tiles [b*T, (b+1)*T) and appends to shard b % kShards; queue-mode
+    // rounds append at most min(V, kQueueCap) entries to any shard.
     const uint64_t npairs = (V + 63) / 64;
-    const uint64_t T = (npairs + grid - 1) / grid;
-    const uint64_t blocks_per_shard = (grid + kShards - 1) / kShards;
+    const uint64_t pg = prepare_grid(grid);
+    const uint64_t T = (npairs + pg - 1) / pg;
+    const uint64_t blocks_per_shard = (pg + kShards - 1) / kShards;
     const uint64_t cap = std::max<uint64_t>(blocks_per_shard * 64 * T, std::min<uint64_t>(V, kQueueCap));
     return (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
 }

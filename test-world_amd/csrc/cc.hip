@@ -195,7 +195,8 @@ static int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots) {
     unsigned long long *acc = g->d_acc;
     int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 2 * kShards, 0u);
     if (rc) return rc;
-    k_count_roots<<<g->grid, kBlock, 0, g->stream>>>(g->V, label, g->d_inv, acc);
+    // a streaming scan: 512 workgroups, as for the reach statistics
+    k_count_roots<<<std::min(g->grid, 512u), kBlock, 0, g->stream>>>(g->V, label, g->d_inv, acc);
     PTG_HIP(hipGetLastError());
     unsigned long long h[kShards];
     PTG_HIP(hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, g->stream));

@@ -203,3 +203,19 @@ def test_generator_matches_golden_fixture():
         np.testing.assert_array_equal(g.cols, z["col"])
         np.testing.assert_array_equal(g.weights, z["w"])
     assert hashlib.sha256(g.cols.tobytes()).hexdigest()
+
+
+def test_device_buffer_views_do_not_own_memory():
+    """A view is a window of its buffer: bounds are checked, and freeing the
+    view never frees the buffer's memory (no device needed: a view is only
+    pointer arithmetic)."""
+    buf = pg.DeviceBuffer.__new__(pg.DeviceBuffer)
+    buf.nbytes, buf.ptr = 4096, 0x10000
+    v = buf.view(8, 100)
+    assert (v.ptr, v.nbytes) == (0x10008, 100)
+    for off, n in ((-4, 8), (4090, 8), (0, 4097)):
+        with pytest.raises(ValueError):
+            buf.view(off, n)
+    v.free()  # must not reach the library
+    assert v.ptr is None and buf.ptr == 0x10000
+    buf.ptr = None  # the fake buffer owns nothing either

@@ -1,8 +1,8 @@
 """ctypes view of the CPU oracle (oracle/oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
-never as the computation under test.
+cpu_baseline leg of bench.py, always to check results against or as the timed
+CPU baseline, never as the computation under test.
 
 Every function takes a graph in the reference layout (ptGraph/main.cu:257-264):
 `ptr` holds V entries and the last vertex's degree is E - ptr[V-1]. The C file

@@ -2,8 +2,9 @@
  * oracle.c -- CPU oracle for the graph hot path (BFS / SSSP / CC / PageRank).
  *
  * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * or the timed CPU baseline; the product path never calls it.
+ * cpu_baseline leg of bench.py may load this library, and only to check
+ * results against or as the timed CPU baseline; the product path never
+ * calls it.
  *
  * Graphs use the reference layout: `ptr` has V entries and the degree of the
  * last vertex is E - ptr[V-1] (ptGraph/main.cu:257-264).

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.c) before results are checked against it.
 
 Each family of the oracle is checked against sources it does not share code
 with: the committed golden fixtures (pure-Python brute force, scipy.csgraph,

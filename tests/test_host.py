@@ -219,3 +219,15 @@ def test_device_buffer_views_do_not_own_memory():
     v.free()  # must not reach the library
     assert v.ptr is None and buf.ptr == 0x10000
     buf.ptr = None  # the fake buffer owns nothing either
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it exits with a message and
+    status 2 before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""

@@ -50,6 +50,12 @@ def lib():
         L.orc_pagerank_f64.argtypes = [u32, u64, _u32p, _u32p, ctypes.c_double, i32, _f64p]
         L.orc_floyd_warshall.argtypes = [u32, u64, _u32p, _u32p, _u32p, u32, u32, _u32p]
         L.orc_closure_min.argtypes = [u32, u64, _u32p, _u32p, _u32p]
+        L.orc_certify_paths.argtypes = [u32, u64, _u32p, _u32p, _u32p, u32, u32, _u32p, i32]
+        L.orc_pagerank_f64_sym.argtypes = [u32, u64, _u32p, _u32p, ctypes.c_double, i32, i32, _f64p]
+        L.orc_degree_pass.argtypes = [u32, u64, _u32p, _u32p]
+        L.orc_degree_pass.restype = ctypes.c_int64
+        L.orc_overload_gather.argtypes = [i32, u32, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p]
+        L.orc_overload_gather.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -163,3 +169,51 @@ def closure_min(ptr, col):
     if lib().orc_closure_min(ptr.size, E, _p(ptr), _p(col), _p(out)):
         raise ValueError("orc_closure_min failed")
     return out
+
+
+def certify_paths(ptr, col, w, src, val, unreached=0xFFFFFFFF, threads=1):
+    """True when `val` is exactly the BFS levels (w None) or SSSP distances from
+    `src` (reference convention, weights >= 1), checked by the shortest-path
+    certificate in oracle.c; raises on bad arguments or a zero weight."""
+    ptr, col, w, E = _graph(ptr, col, w)
+    val = np.ascontiguousarray(val, dtype=np.uint32)
+    if val.size != ptr.size:
+        raise ValueError("certify_paths: val has the wrong length")
+    rc = lib().orc_certify_paths(ptr.size, E, _p(ptr), _p(col), _p(w), src, unreached, _p(val), threads)
+    if rc < 0:
+        raise ValueError("orc_certify_paths: bad arguments")
+    return rc == 0
+
+
+def pagerank_f64_sym(ptr, col, damping=0.85, iters=20, threads=1):
+    """pagerank_f64 for a graph that is symmetric as a multigraph, pulled over
+    the out-lists on `threads` threads (the caller vouches for symmetry)."""
+    ptr, col, _, E = _graph(ptr, col)
+    out = np.empty(ptr.size, dtype=np.float64)
+    if lib().orc_pagerank_f64_sym(ptr.size, E, _p(ptr), _p(col), damping, iters, threads, _p(out, _f64p)):
+        raise ValueError("orc_pagerank_f64_sym failed")
+    return out
+
+
+def degree_pass(ptr, E):
+    """The reference's degree loop; returns (degrees, ns)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.uint32)
+    deg = np.zeros(ptr.size, dtype=np.uint32)
+    ns = lib().orc_degree_pass(ptr.size, E, _p(ptr), _p(deg))
+    return deg, int(ns)
+
+
+def overload_gather(ptr, col, degree, active, threads=20):
+    """The reference's overload gather of the adjacency of `active` with its
+    thread fan-out; returns (gathered edges, ns)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    degree = np.ascontiguousarray(degree, dtype=np.uint32)
+    active = np.ascontiguousarray(active, dtype=np.uint32)
+    d = degree[active].astype(np.uint64)
+    aptr = np.concatenate([[0], np.cumsum(d)[:-1]]).astype(np.uint32) if active.size else np.zeros(1, np.uint32)
+    total = int(d.sum())
+    out = np.empty(max(total, 1), dtype=np.uint32)
+    ns = lib().orc_overload_gather(threads, active.size, _p(degree), _p(aptr), _p(ptr),
+                                   _p(active if active.size else np.zeros(1, np.uint32)), _p(col), _p(out))
+    return out[:total], int(ns)

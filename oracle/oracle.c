@@ -22,15 +22,26 @@
  *   2. Textbook algorithms: queue BFS, binary-heap Dijkstra with uint64
  *      accumulation, reachability-min labels, union-find CC, FP64 PageRank.
  *   3. Brute force on tiny graphs: Floyd-Warshall and transitive closure.
+ *   4. Full-size helpers, T pthreads: a shortest-path certificate checker
+ *      (exact BFS levels / SSSP distances without recomputing them), FP64
+ *      PageRank pulled over the out-lists of a symmetric graph, and the
+ *      reference's host loops timed as written: the degree pass
+ *      (ptGraph/main.cu:1703-1710) and the overload gather bfsDynamic with its
+ *      thread fan-out (ptGraph/main.cu:1126-1156, :1915-1938).
  *
  * Pinning (tests/test_oracle.py): family 1 == family 2 == family 3 == scipy's
  * csgraph routines on random and hand-made graphs, closed forms on paths,
  * cycles, stars and complete graphs, and the committed golden fixtures.
  * PageRank (no arithmetic in the reference, ptGraph/main.cu:891-959) is pinned
  * against a dense numpy iteration and closed forms (uniform rank on regular
- * graphs, 1/V on edgeless graphs, total mass 1).
+ * graphs, 1/V on edgeless graphs, total mass 1). Family 4: the certificate
+ * accepts the textbook outputs and rejects planted mistakes (a level or
+ * distance moved by one, a reached vertex dropped, an unreachable one
+ * reached); the symmetric PageRank equals the push form to 1e-12; the degree
+ * pass equals the closed form and the gather equals the concatenated lists.
  */
 #define _GNU_SOURCE
+#include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stdint.h>
@@ -524,4 +535,233 @@ int orc_closure_min(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t 
             }
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 4. Full-size checkers and the reference's host loops (T threads)           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint32_t V;
+    uint64_t E;
+    const uint32_t *ptr, *col, *w;
+    const uint32_t *val;
+    uint32_t unreached;
+    _Atomic uint8_t *tight;
+    _Atomic int bad;
+    const double *contrib;
+    double base, damping;
+    double *next;
+    int nthreads;
+} mt_ctx;
+
+typedef struct {
+    mt_ctx *c;
+    int tid;
+} mt_arg;
+
+static void run_threads(mt_ctx *c, void *(*fn)(void *)) {
+    int T = c->nthreads < 1 ? 1 : c->nthreads;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * T);
+    mt_arg *a = (mt_arg *)malloc(sizeof(mt_arg) * T);
+    for (int t = 0; t < T; t++) {
+        a[t].c = c;
+        a[t].tid = t;
+        if (t) pthread_create(&th[t], NULL, fn, &a[t]);
+    }
+    fn(&a[0]);
+    for (int t = 1; t < T; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(a);
+}
+
+/* Edge pass of the shortest-path certificate: for every edge (u, v) out of a
+ * reached u, v is reached and val[v] <= val[u] + w; v gets a "tight" mark
+ * when val[v] == val[u] + w. */
+static void *cert_edges(void *p) {
+    mt_arg *a = (mt_arg *)p;
+    mt_ctx *c = a->c;
+    uint64_t b, e;
+    range_of(c->V, a->tid, c->nthreads, &b, &e);
+    for (uint64_t u = b; u < e && !atomic_load_explicit(&c->bad, memory_order_relaxed); u++) {
+        uint64_t vu = c->val[u];
+        if (vu == c->unreached) continue;
+        uint32_t d = deg_of(c->V, c->E, c->ptr, (uint32_t)u);
+        for (uint32_t k = 0; k < d; k++) {
+            uint64_t x = (uint64_t)c->ptr[u] + k;
+            uint32_t v = c->col[x];
+            uint64_t cand = vu + (c->w ? c->w[x] : 1u);
+            uint64_t vv = c->val[v];
+            if (vv == c->unreached || vv > cand) {
+                atomic_store_explicit(&c->bad, 1, memory_order_relaxed);
+                break;
+            }
+            if (vv == cand) atomic_store_explicit(&c->tight[v], 1, memory_order_relaxed);
+        }
+    }
+    return NULL;
+}
+
+/*
+ * Checks that `val` is exactly the BFS levels (w == NULL) or the SSSP
+ * distances from `src` in the reference's output convention (val[src] = 1,
+ * `unreached` for vertices src cannot reach), without computing them:
+ *   (1) val[src] == 1;
+ *   (2) every edge (u, v) out of a reached u has v reached and
+ *       val[v] <= val[u] + w(u, v);
+ *   (3) every reached v != src has an edge (u, v) with val[u] + w == val[v].
+ * With every weight >= 1 these hold for the true distances and only for them:
+ * (2) gives val <= true distance along every path from src, and from any
+ * reached v the tight edges of (3) lead back, strictly decreasing, to src, so
+ * val[v] is the length of a real path. Zero weights would let a tight cycle
+ * stand in for a path, so they are refused.
+ * Returns 0 if `val` passes, 1 if it fails, -1 on bad arguments (including a
+ * zero weight).
+ */
+int orc_certify_paths(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *col,
+                      const uint32_t *w, uint32_t src, uint32_t unreached, const uint32_t *val,
+                      int threads) {
+    if (V == 0 || src >= V || !val || !ptr || (E && !col)) return -1;
+    if (w)
+        for (uint64_t x = 0; x < E; x++)
+            if (w[x] == 0) return -1;
+    if (val[src] != 1) return 1;
+    mt_ctx c;
+    memset(&c, 0, sizeof c);
+    c.V = V;
+    c.E = E;
+    c.ptr = ptr;
+    c.col = col;
+    c.w = w;
+    c.val = val;
+    c.unreached = unreached;
+    c.nthreads = threads < 1 ? 1 : threads;
+    c.tight = (_Atomic uint8_t *)calloc(V, 1);
+    if (!c.tight) return -1;
+    atomic_init(&c.bad, 0);
+    run_threads(&c, cert_edges);
+    int bad = atomic_load(&c.bad);
+    for (uint32_t v = 0; v < V && !bad; v++)
+        if (v != src && val[v] != unreached && !atomic_load_explicit(&c.tight[v], memory_order_relaxed))
+            bad = 1;
+    free((void *)c.tight);
+    return bad;
+}
+
+static void *pr_pull(void *p) {
+    mt_arg *a = (mt_arg *)p;
+    mt_ctx *c = a->c;
+    uint64_t b, e;
+    range_of(c->V, a->tid, c->nthreads, &b, &e);
+    for (uint64_t v = b; v < e; v++) {
+        uint32_t d = deg_of(c->V, c->E, c->ptr, (uint32_t)v);
+        double s = 0.0;
+        for (uint32_t k = 0; k < d; k++) s += c->contrib[c->col[(uint64_t)c->ptr[v] + k]];
+        c->next[v] = c->base + c->damping * s;
+    }
+    return NULL;
+}
+
+/* orc_pagerank_f64 for a graph that is symmetric as a multigraph (every pair
+ * (u, v) occurs as often as (v, u)): the in-list of v is then its out-list, so
+ * each vertex pulls r(u)/outdeg(u) over its own list, one thread per vertex
+ * range. Same definition as orc_pagerank_f64; the caller vouches for symmetry. */
+int orc_pagerank_f64_sym(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32_t *col,
+                         double damping, int iters, int threads, double *out) {
+    if (V == 0 || !out || iters < 0) return -1;
+    double *contrib = (double *)malloc(sizeof(double) * V);
+    if (!contrib) return -1;
+    mt_ctx c;
+    memset(&c, 0, sizeof c);
+    c.V = V;
+    c.E = E;
+    c.ptr = ptr;
+    c.col = col;
+    c.contrib = contrib;
+    c.damping = damping;
+    c.next = out;
+    c.nthreads = threads < 1 ? 1 : threads;
+    for (uint32_t v = 0; v < V; v++) out[v] = 1.0 / V;
+    for (int it = 0; it < iters; it++) {
+        double dang = 0.0;
+        for (uint32_t u = 0; u < V; u++) {
+            uint32_t d = deg_of(V, E, ptr, u);
+            if (d == 0) {
+                dang += out[u];
+                contrib[u] = 0.0;
+            } else {
+                contrib[u] = out[u] / d;
+            }
+        }
+        c.base = (1.0 - damping) / V + damping * dang / V;
+        run_threads(&c, pr_pull);
+    }
+    free(contrib);
+    return 0;
+}
+
+/* The reference's degree pass as written (ptGraph/main.cu:1703-1710):
+ * degree[i] = ptr[i+1] - ptr[i] for i < V-1, stopping at an offset beyond E,
+ * then degree[V-1] = E - ptr[V-1]. Returns the elapsed ns. */
+int64_t orc_degree_pass(uint32_t V, uint64_t E, const uint32_t *ptr, uint32_t *degree) {
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (uint32_t i = 0; i + 1 < V; i++) {
+        if (ptr[i] > E) break;
+        degree[i] = ptr[i + 1] - ptr[i];
+    }
+    degree[V - 1] = (uint32_t)(E - ptr[V - 1]);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    return (int64_t)(t1.tv_sec - t0.tv_sec) * 1000000000LL + (t1.tv_nsec - t0.tv_nsec);
+}
+
+typedef struct {
+    int tid, nthreads;
+    uint32_t begin, n_active;
+    const uint32_t *degree, *active_ptr, *ptr, *active, *col;
+    uint32_t *gathered;
+} gather_arg;
+
+/* One worker of the overload gather, ptGraph/main.cu:1126-1156 (bfsDynamic):
+ * chunk = ceil((n_active - begin) / T), worker t copies the adjacency of
+ * active[t*chunk + begin .. min(+chunk, n_active)) to active_ptr[i]. */
+static void *gather_worker(void *p) {
+    gather_arg *a = (gather_arg *)p;
+    /* float arithmetic as in the reference (exact below 2^24 vertices) */
+    float todo = (float)(a->n_active - a->begin);
+    uint32_t chunk = (uint32_t)ceilf(todo / (float)a->nthreads);
+    uint64_t left = (uint64_t)a->tid * chunk + a->begin;
+    uint64_t right = left + chunk < a->n_active ? left + chunk : a->n_active;
+    for (uint64_t i = left; i < right; i++) {
+        uint32_t v = a->active[i];
+        uint32_t d = a->degree[v];
+        uint64_t here = a->active_ptr[i], there = a->ptr[v];
+        for (uint32_t j = 0; j < d; j++) a->gathered[here + j] = a->col[there + j];
+    }
+    return NULL;
+}
+
+/* The per-iteration host gather of the reference's swap path
+ * (ptGraph/main.cu:1915-1938): T = 20 threads, or 1 below 50 overload
+ * vertices, created and joined for this one gather; `active_ptr` holds the
+ * exclusive prefix of the overload vertices' degrees. Returns elapsed ns
+ * including thread creation and join, as the reference's cpu timer does. */
+int64_t orc_overload_gather(int threads, uint32_t n_active, const uint32_t *degree,
+                            const uint32_t *active_ptr, const uint32_t *ptr,
+                            const uint32_t *active, const uint32_t *col, uint32_t *gathered) {
+    struct timespec t0, t1;
+    int T = threads < 1 ? 1 : threads;
+    if (n_active < 50) T = 1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * T);
+    gather_arg *a = (gather_arg *)malloc(sizeof(gather_arg) * T);
+    for (int t = 0; t < T; t++) {
+        a[t] = (gather_arg){t, T, 0, n_active, degree, active_ptr, ptr, active, col, gathered};
+        pthread_create(&th[t], NULL, gather_worker, &a[t]);
+    }
+    for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(th);
+    free(a);
+    return (int64_t)(t1.tv_sec - t0.tv_sec) * 1000000000LL + (t1.tv_nsec - t0.tv_nsec);
 }

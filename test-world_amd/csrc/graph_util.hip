@@ -1,9 +1,12 @@
 // Small device utilities: fills, the result un-permutation, the one-time
-// symmetry check and reached-set statistics.
+// symmetry fingerprint and reached-set statistics.
 #include <string.h>
 
 #include <algorithm>
+#include <random>
+#include <vector>
 
+#include "fingerprint.h"
 #include "static_expand.h"
 
 namespace ptg {
@@ -104,57 +107,90 @@ int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst) {
 }
 
 // ---------------------------------------------------------------------------
-// Symmetry check (once per graph, at creation): every adjacency sorted
-// ascending and every edge (v, u) matched by (u, v), found by binary search.
+// Symmetry check (once per graph, at creation): the multiset fingerprints of
+// the pairs (v, u) and (u, v) must agree (fingerprint.h). One pass over the
+// static degree bins reads every column once; each workgroup leaves its four
+// partial products, which the host multiplies.
 // ---------------------------------------------------------------------------
 
-struct CheckSorted {
-    const uint32_t *__restrict__ off;
+struct FpOp {
     const uint32_t *__restrict__ col;
-    uint32_t *flag;
-    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return off[v + 1]; }
-    __device__ __forceinline__ void edge(bool valid, uint32_t end, uint64_t e) const {
-        if (valid && e + 1 < end && col[e] > col[e + 1]) *flag = 0u;
+    FpKeys k;
+    mutable FpAcc a;
+    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return v; }
+    __device__ __forceinline__ void edge(bool valid, uint32_t v, uint64_t e) const {
+        if (valid) fp_edge(k, a, v, col[e]);
     }
 };
 
-struct CheckReverse {
-    const uint32_t *__restrict__ off;
-    const uint32_t *__restrict__ col;
-    uint32_t *flag;
-    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return v; }
-    __device__ __forceinline__ void edge(bool valid, uint32_t v, uint64_t e) const {
-        if (!valid) return;
-        const uint32_t u = col[e];
-        uint32_t lo = off[u], hi = off[u + 1];
-        while (lo < hi) {
-            const uint32_t mid = lo + ((hi - lo) >> 1);
-            if (col[mid] < v) lo = mid + 1; else hi = mid;
-        }
-        if (lo == off[u + 1] || col[lo] != v) *flag = 0u;
+__device__ __forceinline__ uint64_t wave_fp_product(uint64_t x) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, d, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), d, 64);
+        x = fp_mul(x, (uint64_t)hi << 32 | lo);
     }
-};
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fingerprint(Frontier in, const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ col, FpKeys k,
+                                                        unsigned long long *__restrict__ part) {
+    __shared__ uint64_t s[4][kWavesPerBlock];
+    FpOp op{col, k, FpAcc{{1, 1}, {1, 1}}};
+    expand_static(in, off, op);
+    const uint64_t x[4] = {op.a.f[0], op.a.f[1], op.a.b[0], op.a.b[1]};
+    const uint32_t wib = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint64_t y = wave_fp_product(x[i]);
+        if (lane_id() == 0) s[i][wib] = y;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        uint64_t p = 1;
+        for (int w = 0; w < kWavesPerBlock; w++) p = fp_mul(p, s[threadIdx.x][w]);
+        part[4ull * blockIdx.x + threadIdx.x] = p;
+    }
+}
 
 int check_symmetric(ptg_graph *g, int *sym) {
     if (g->symmetric >= 0) {
         *sym = g->symmetric;
         return PTG_OK;
     }
-    uint32_t *flag = g->d_bits[4];
-    int rc = fill_u32(g, flag, 2, 1u);
-    if (rc) return rc;
+    const uint32_t blocks = g->grid;
+    unsigned long long *part = nullptr;
+    std::vector<unsigned long long> h(4ull * blocks);
+    PTG_HIP(hipMalloc((void **)&part, h.size() * sizeof(unsigned long long)));
     const Frontier fin = g->out.bins.q.with(g->out.bins.d_ctr);
-    k_expand<CheckSorted><<<g->grid, kBlock, 0, g->stream>>>(fin, g->out.off, CheckSorted{g->out.off, g->out.col, flag});
-    k_expand<CheckReverse><<<g->grid, kBlock, 0, g->stream>>>(fin, g->out.off,
-                                                              CheckReverse{g->out.off, g->out.col, flag + 1});
-    PTG_HIP(hipGetLastError());
-    uint32_t h[2];
-    PTG_HIP(hipMemcpyAsync(h, flag, sizeof h, hipMemcpyDeviceToHost, g->stream));
-    PTG_HIP(hipStreamSynchronize(g->stream));
-    // The reverse search assumes sorted lists; unsorted means "not proven".
-    g->symmetric = (h[0] && h[1]) ? 1 : 0;
+    k_fingerprint<<<blocks, kBlock, 0, g->stream>>>(fin, g->out.off, g->out.col, fp_random_keys(), part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, h.size() * sizeof(unsigned long long),
+                                            hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return hip_fail(e, "symmetry fingerprint");
+    uint64_t p[4] = {1, 1, 1, 1};
+    for (uint32_t b = 0; b < blocks; b++)
+        for (int i = 0; i < 4; i++) p[i] = fp_mul(p[i], h[4ull * b + i]);
+    g->symmetric = (p[0] == p[2] && p[1] == p[3]) ? 1 : 0;
     *sym = g->symmetric;
     return PTG_OK;
+}
+
+FpKeys fp_random_keys() {
+    std::random_device rd;
+    FpKeys k{};
+    auto draw = [&rd]() {
+        uint64_t x = 0;
+        while (x == 0) x = fp_reduce(((uint64_t)rd() << 32 | rd()) & ((1ull << 62) - 1));
+        return x;
+    };
+    for (int i = 0; i < 2; i++) {
+        k.r[i] = draw();
+        k.s[i] = draw();
+    }
+    return k;
 }
 
 // ---------------------------------------------------------------------------

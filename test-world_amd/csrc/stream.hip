@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "device_util.h"
+#include "fingerprint.h"
 #include "relabel.h"
 
 namespace ptg {
@@ -382,43 +383,30 @@ static void plan_chunks(uint32_t V, const uint32_t *off, uint64_t max_edges, std
     }
 }
 
-// Fingerprint test of symmetry over the host CSR: equal in- and out-degrees
-// and equal sums of an asymmetric 64-bit hash over (u, v) and (v, u).
+// Symmetry of the host CSR as a multigraph: the multiset fingerprints of the
+// pairs (u, v) and (v, u) agree (fingerprint.h), products taken per thread
+// over a vertex range and then multiplied.
 static bool host_symmetric(uint32_t V, const uint32_t *off, const uint32_t *col) {
     const int T = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    auto h = [](uint64_t x) {
-        x += 0x9E3779B97F4A7C15ull;
-        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-        return x ^ (x >> 31);
-    };
-    std::vector<unsigned long long> fwd(T, 0), bwd(T, 0);
-    std::vector<uint32_t> in((size_t)V, 0);
+    const FpKeys keys = fp_random_keys();
+    std::vector<FpAcc> acc(T, FpAcc{{1, 1}, {1, 1}});
     std::vector<std::thread> th;
     for (int t = 0; t < T; t++)
         th.emplace_back([&, t] {
             const uint32_t b = (uint32_t)((uint64_t)V * t / T), e = (uint32_t)((uint64_t)V * (t + 1) / T);
-            unsigned long long f = 0, r = 0;
+            FpAcc a{{1, 1}, {1, 1}};
             for (uint32_t u = b; u < e; u++)
-                for (uint32_t k = off[u]; k < off[u + 1]; k++) {
-                    const uint32_t v = col[k];
-                    f += h(((uint64_t)u << 32) | v);
-                    r += h(((uint64_t)v << 32) | u);
-                    __atomic_fetch_add(&in[v], 1u, __ATOMIC_RELAXED);
-                }
-            fwd[t] = f;
-            bwd[t] = r;
+                for (uint32_t k = off[u]; k < off[u + 1]; k++) fp_edge(keys, a, u, col[k]);
+            acc[t] = a;
         });
     for (auto &x : th) x.join();
-    unsigned long long F = 0, R = 0;
-    for (int t = 0; t < T; t++) {
-        F += fwd[t];
-        R += bwd[t];
-    }
-    if (F != R) return false;
-    for (uint32_t u = 0; u < V; u++)
-        if (in[u] != off[u + 1] - off[u]) return false;
-    return true;
+    FpAcc all{{1, 1}, {1, 1}};
+    for (const FpAcc &a : acc)
+        for (int i = 0; i < 2; i++) {
+            all.f[i] = fp_mul(all.f[i], a.f[i]);
+            all.b[i] = fp_mul(all.b[i], a.b[i]);
+        }
+    return all.f[0] == all.b[0] && all.f[1] == all.b[1];
 }
 
 int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges) {

@@ -516,3 +516,50 @@ def test_cc_component_shapes(flags):
                 lab, st = g.cc()
                 np.testing.assert_array_equal(lab, want, err_msg=name)
                 assert st.reached == len(np.unique(want))
+
+
+# Repeat counts matter to PageRank: 0->1 twice against 1->0 once has every
+# reverse edge present, but it is not symmetric as a multigraph, so the pull
+# must use the transposed lists (vertex 1 gets 0's contribution twice).
+MULTI_UNEQUAL = ([0, 2, 4, 5], [1, 1, 0, 2, 1])       # adj 0:[1,1] 1:[0,2] 2:[1]
+MULTI_EQUAL = ([0, 3, 5, 6], [1, 2, 1, 0, 0, 0])      # adj 0:[1,2,1] 1:[0,0] 2:[0]
+
+
+@pytest.mark.parametrize("case", ["unequal", "equal"])
+@pytest.mark.parametrize("mode", ["relabel", "no_relabel", "stream"])
+def test_repeat_counts_and_symmetry(case, mode):
+    ptr, col = MULTI_UNEQUAL if case == "unequal" else MULTI_EQUAL
+    ptr = np.array(ptr[:-1], np.uint32)
+    col = np.array(col, np.uint32)
+    w = np.arange(1, col.size + 1, dtype=np.uint32)
+    flags = {"relabel": 0, "no_relabel": pg.FLAG_NO_RELABEL, "stream": pg.FLAG_STREAM}[mode]
+    say("case repeat counts %s %s" % (case, mode))
+    cfg = pg.default_config(flags=flags)
+    with pg.Graph(ptr, col, None if mode == "stream" else w, config=cfg) as g:
+        assert g.is_symmetric() == (case == "equal")
+        r, _ = g.pagerank(0.85, 20)
+        assert pr_l1(ptr, col, r) <= PR_L1
+        for s in range(ptr.size):
+            np.testing.assert_array_equal(g.bfs(s)[0], oracle.bfs(ptr, col, s)[0])
+        if mode != "stream":
+            lab, _ = g.cc()
+            np.testing.assert_array_equal(lab, oracle.cc_reach_min(ptr, col))
+
+
+def test_repeat_counts_rmat_pagerank():
+    """R-MAT-12 with every edge of vertex 3 doubled in one direction only:
+    the resident and streamed PageRank both follow the FP64 oracle."""
+    csr = pg.gen_rmat(12, 16, seed=1)
+    ptr, col = csr.node_pointers.astype(np.int64), csr.cols
+    E = col.size
+    nxt = np.append(ptr[1:], E)
+    lists = [col[ptr[u]:nxt[u]] for u in range(ptr.size)]
+    lists[3] = np.repeat(lists[3], 2)
+    col2 = np.concatenate(lists).astype(np.uint32)
+    ptr2 = np.concatenate([[0], np.cumsum([l.size for l in lists])[:-1]]).astype(np.uint32)
+    say("case repeat counts rmat12")
+    for flags in (0, pg.FLAG_NO_RELABEL, pg.FLAG_STREAM):
+        with pg.Graph(ptr2, col2, config=pg.default_config(flags=flags)) as g:
+            assert not g.is_symmetric()
+            r, _ = g.pagerank(0.85, 20)
+            assert pr_l1(ptr2, col2, r) <= PR_L1

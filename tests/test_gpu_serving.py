@@ -1,7 +1,9 @@
 """GPU tests of the serving loop (ptg_server_*): every result served in a mix
-of concurrent, batched and coalesced requests equals the same query run
-alone (bit for bit, PageRank included: its sums run in a fixed order), with
-memory admission evicting, waiting and rejecting as configured."""
+of concurrent, batched and coalesced requests equals the oracle's (BFS queue
+levels, Dijkstra distances and reach-min labels bit for bit, PageRank within
+1e-6 L1 of the FP64 oracle) and the same query run alone (bit for bit,
+PageRank included: its sums run in a fixed order), with memory admission
+evicting, waiting and rejecting as configured."""
 import os
 import sys
 
@@ -14,6 +16,7 @@ import ptgraph as pg
 pytestmark = pytest.mark.gpu
 
 MIX = ((pg.Q_BFS, 0.50), (pg.Q_SSSP, 0.30), (pg.Q_CC, 0.15), (pg.Q_PAGERANK, 0.05))
+PR_L1 = 1e-6
 
 
 def say(msg):
@@ -61,7 +64,21 @@ def alone(graphs, r):
     return g.pagerank(r.damping, r.iterations)[0].view(np.uint32)
 
 
-def check(reqs, comps, outs, ref_graphs, srv=None):
+def against_oracle(csr, r, got):
+    """The served (or alone) result of request `r` against the oracle."""
+    ptr, col = csr.node_pointers, csr.cols
+    if r.kind == pg.Q_BFS:
+        np.testing.assert_array_equal(got, oracle.bfs(ptr, col, r.source)[0])
+    elif r.kind == pg.Q_SSSP:
+        np.testing.assert_array_equal(got, oracle.dijkstra(ptr, col, csr.weights, r.source))
+    elif r.kind == pg.Q_CC:
+        np.testing.assert_array_equal(got, oracle.cc_reach_min(ptr, col))
+    else:
+        want = oracle.pagerank_f64(ptr, col, r.damping, r.iterations)
+        assert float(np.abs(got.view(np.float32).astype(np.float64) - want).sum()) <= PR_L1
+
+
+def check(reqs, comps, outs, ref_graphs, srv=None, csrs=None):
     assert len(comps) == len(reqs)
     if srv is not None:
         assert srv.error() == "", srv.error()
@@ -73,10 +90,14 @@ def check(reqs, comps, outs, ref_graphs, srv=None):
         key = (r.graph, r.kind, r.source if r.kind in (pg.Q_BFS, pg.Q_SSSP) else 0)
         if key not in cache:
             cache[key] = alone(ref_graphs, r)
+            if csrs is not None:
+                against_oracle(csrs[r.graph], r, cache[key])
         want = cache[key]
         assert c.checksum == pg.checksum(want), "checksum tag %d kind %d" % (r.tag, r.kind)
         if outs is not None:
             np.testing.assert_array_equal(outs[r.tag], want, err_msg="tag %d kind %d" % (r.tag, r.kind))
+            if csrs is not None and r.kind != pg.Q_PAGERANK:
+                against_oracle(csrs[r.graph], r, outs[r.tag])
         assert c.submit_ns <= c.start_ns <= c.done_ns
 
 
@@ -97,7 +118,7 @@ def test_mix_equals_alone_burst_and_replay(csrs):
             srv.pause(False)
             srv.drain()
             comps = srv.completions()
-            check(reqs, comps, outs, ref, srv)
+            check(reqs, comps, outs, ref, srv, csrs)
             s = srv.summary()
             assert s.completed == 400 and s.rejected == 0
             assert s.mean_batch > 1.5  # a burst is served in batches
@@ -111,7 +132,7 @@ def test_mix_equals_alone_burst_and_replay(csrs):
             arrivals = np.cumsum(rng.exponential(1e9 / 20000, 300)).astype(np.uint64)
             srv.replay(reqs2, arrivals)
             srv.drain()
-            check(reqs2, srv.completions(), outs_by_tag, ref, srv)
+            check(reqs2, srv.completions(), outs_by_tag, ref, srv, csrs)
             assert srv.summary().p99_ns >= srv.summary().p50_ns > 0
     finally:
         for g in ref + graphs:
@@ -133,7 +154,7 @@ def test_plain_mode_serves_one_at_a_time(csrs):
             srv.drain()
             comps = srv.completions()
             assert all(c.batch_size == 1 for c in comps)
-            check(reqs, comps, None, ref, srv)
+            check(reqs, comps, None, ref, srv, csrs)
             # one at a time: launch intervals do not overlap
             iv = sorted((c.start_ns, c.done_ns) for c in comps)
             assert all(b[0] >= a[1] for a, b in zip(iv, iv[1:]))
@@ -166,7 +187,7 @@ def test_memory_admission_evicts_and_rejects(csrs):
             srv.pause(False)
             srv.drain()
             comps = srv.completions()
-            check(reqs, comps, None, ref, srv)
+            check(reqs, comps, None, ref, srv, csrs)
             s = srv.summary()
             info = "launches %d batch %.1f peak %d evictions %d" % (s.launches, s.mean_batch, s.state_bytes_peak,
                                                                    s.evictions)

@@ -215,3 +215,99 @@ def test_rmat16_golden_hashes():
     assert iters == int(meta["max_level"])
     _, _, reached, edges = oracle.bfs(g.node_pointers, g.cols, 0)
     assert reached == int(meta["reached"]) and edges == int(meta["edges_traversed"])
+
+
+# --- family 4: full-size helpers -------------------------------------------
+
+def planted_mistakes(rng, val, src, unreached, step):
+    """Copies of a correct `val` with one plausible mistake each."""
+    reached = np.nonzero((val != unreached) & (np.arange(val.size) != src))[0]
+    unr = np.nonzero(val == unreached)[0]
+    out = []
+    if reached.size:
+        for delta in (step, -1):
+            v = int(rng.choice(reached))
+            b = val.copy()
+            b[v] = np.uint32(int(b[v]) + delta)
+            out.append(("shift %+d at %d" % (delta, v), b))
+        b = val.copy()
+        b[int(rng.choice(reached))] = unreached
+        out.append(("dropped", b))
+    if unr.size:
+        b = val.copy()
+        b[int(rng.choice(unr))] = np.uint32(int(val[val != unreached].max()) + 1)
+        out.append(("unreachable reached", b))
+    b = val.copy()
+    b[src] = 2
+    out.append(("source", b))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_certificate_accepts_exact_and_rejects_planted(seed):
+    rng = np.random.default_rng(100 + seed)
+    V = int(rng.integers(2, 60))
+    ptr, col, w = random_graph(rng, V, float(rng.uniform(0.03, 0.2)), bool(seed % 2), 9)
+    w = w + 1  # the certificate needs weights >= 1
+    for s in range(0, V, max(1, V // 4)):
+        lv = oracle.bfs(ptr, col, s)[0]
+        assert oracle.certify_paths(ptr, col, None, s, lv, threads=1 + seed % 4)
+        for name, bad in planted_mistakes(rng, lv, s, 0xFFFFFFFF, 1):
+            assert not oracle.certify_paths(ptr, col, None, s, bad, threads=2), name
+        d = oracle.dijkstra(ptr, col, w, s, SENT)
+        assert oracle.certify_paths(ptr, col, w, s, d, SENT, threads=3)
+        for name, bad in planted_mistakes(rng, d, s, SENT, 1):
+            assert not oracle.certify_paths(ptr, col, w, s, bad, SENT), name
+
+
+def test_certificate_refuses_zero_weights():
+    ptr, col = path_graph(4)
+    with pytest.raises(ValueError):
+        oracle.certify_paths(ptr, col, np.zeros(col.size, np.uint32), 0, np.arange(1, 5, dtype=np.uint32))
+
+
+def test_certificate_on_golden_fixtures():
+    for path in HAND + RMAT:
+        f = load(path)
+        for i, s in enumerate(f["srcs"]):
+            assert oracle.certify_paths(f["ptr"], f["col"], None, int(s), f["bfs"][i], threads=2)
+            if (f["w"] > 0).all():
+                assert oracle.certify_paths(f["ptr"], f["col"], f["w"], int(s), f["sssp"][i], SENT, threads=2)
+
+
+@pytest.mark.parametrize("path", HAND + RMAT, ids=os.path.basename)
+def test_pagerank_sym_equals_push_form(path):
+    f = load(path)
+    ptr, col = f["ptr"], f["col"]
+    if not is_symmetric(ptr, col):
+        pytest.skip("the pull over out-lists needs a symmetric graph")
+    want = oracle.pagerank_f64(ptr, col, 0.85, 20)
+    for t in (1, 3, 8):
+        np.testing.assert_allclose(oracle.pagerank_f64_sym(ptr, col, 0.85, 20, t), want, rtol=1e-12, atol=1e-16)
+    np.testing.assert_allclose(oracle.pagerank_f64_sym(ptr, col, 0.85, 20, 2), f["pr"], rtol=1e-12, atol=1e-16)
+
+
+def test_pagerank_sym_multigraph():
+    # equal repeat counts both ways: 0-1 twice, 0-2 once
+    ptr = np.array([0, 3, 5], np.uint32)
+    col = np.array([1, 2, 1, 0, 0, 0], np.uint32)
+    np.testing.assert_allclose(oracle.pagerank_f64_sym(ptr, col, 0.85, 20, 2), oracle.pagerank_f64(ptr, col, 0.85, 20),
+                               rtol=1e-13)
+
+
+def test_degree_pass_and_overload_gather():
+    import ptgraph as pg
+
+    g = pg.gen_rmat(12, 16, seed=1)
+    ptr, col = g.node_pointers, g.cols
+    E = col.size
+    deg, ns = oracle.degree_pass(ptr, E)
+    np.testing.assert_array_equal(deg, np.diff(np.append(ptr.astype(np.int64), E)))
+    assert ns >= 0
+    rng = np.random.default_rng(5)
+    off = np.append(ptr.astype(np.int64), E)
+    for n, T in ((0, 20), (7, 20), (49, 20), (50, 20), (1000, 20), (3000, 3)):
+        active = np.sort(rng.choice(ptr.size, n, replace=False)).astype(np.uint32)
+        got, _ = oracle.overload_gather(ptr, col, deg, active, threads=T)
+        want = np.concatenate([col[off[v]:off[v + 1]] for v in active]) if n else np.zeros(0, np.uint32)
+        np.testing.assert_array_equal(got, want)

@@ -537,8 +537,14 @@ def test_repeat_counts_and_symmetry(case, mode):
     cfg = pg.default_config(flags=flags)
     with pg.Graph(ptr, col, None if mode == "stream" else w, config=cfg) as g:
         assert g.is_symmetric() == (case == "equal")
-        r, _ = g.pagerank(0.85, 20)
-        assert pr_l1(ptr, col, r) <= PR_L1
+        if mode == "stream" and case == "unequal":
+            # streamed PageRank pulls over the out-lists: refused, not wrong
+            with pytest.raises(pg.PtgError) as e:
+                g.pagerank(0.85, 20)
+            assert e.value.code == pg.ERR_UNSUPPORTED
+        else:
+            r, _ = g.pagerank(0.85, 20)
+            assert pr_l1(ptr, col, r) <= PR_L1
         for s in range(ptr.size):
             np.testing.assert_array_equal(g.bfs(s)[0], oracle.bfs(ptr, col, s)[0])
         if mode != "stream":
@@ -548,7 +554,8 @@ def test_repeat_counts_and_symmetry(case, mode):
 
 def test_repeat_counts_rmat_pagerank():
     """R-MAT-12 with every edge of vertex 3 doubled in one direction only:
-    the resident and streamed PageRank both follow the FP64 oracle."""
+    the resident PageRank follows the FP64 oracle (with and without
+    relabeling), and the streamed graph refuses PageRank."""
     csr = pg.gen_rmat(12, 16, seed=1)
     ptr, col = csr.node_pointers.astype(np.int64), csr.cols
     E = col.size
@@ -561,5 +568,10 @@ def test_repeat_counts_rmat_pagerank():
     for flags in (0, pg.FLAG_NO_RELABEL, pg.FLAG_STREAM):
         with pg.Graph(ptr2, col2, config=pg.default_config(flags=flags)) as g:
             assert not g.is_symmetric()
+            if flags == pg.FLAG_STREAM:
+                with pytest.raises(pg.PtgError) as e:
+                    g.pagerank(0.85, 20)
+                assert e.value.code == pg.ERR_UNSUPPORTED
+                continue
             r, _ = g.pagerank(0.85, 20)
             assert pr_l1(ptr2, col2, r) <= PR_L1

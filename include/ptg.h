@@ -204,6 +204,19 @@ PTG_API int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st);
  * D = rank mass of vertices with outdeg 0; FP32 with compensated sums. */
 PTG_API int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out,
                  ptg_stats *st);
+/* ptg_pagerank that allocates nothing. No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-iteration buffers (2 x 4V bytes of contributions plus partials) live in
+ * one fixed 64 MiB device array per GPU, so V is limited to about 8.3 M
+ * (PTG_ERR_BUDGET beyond). ranks_out must be device memory
+ * (PTG_ERR_INVALID_ARG otherwise); a streamed graph, or a graph that is not
+ * symmetric and has not yet run ptg_pagerank (which builds its in-edges),
+ * gives PTG_ERR_UNSUPPORTED. Per-launch timing is never recorded. Results are
+ * bit-identical to ptg_pagerank's. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race. */
+PTG_API int ptg_pagerank_noalloc(ptg_graph *g, float damping, int iterations, float *ranks_out,
+                                 ptg_stats *st);
 
 /* ---- batched multi-source queries ------------------------------------------ */
 /* Up to 64 sources in one batch (no reference counterpart: the reference runs

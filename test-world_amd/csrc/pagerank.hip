@@ -16,6 +16,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "device_util.h"
@@ -268,12 +270,15 @@ static uint32_t combine_blocks(const ptg_graph *g) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kMaxCombineBlocks, (nl + kWavesPerBlock - 1) / kWavesPerBlock));
 }
 
+// dangling and base partials: 2 iterations x (gather, combine) + base + D0
+static uint64_t pr_part_floats(const ptg_graph *g) { return 2 * (2ull * g->grid + 2ull * kMaxCombineBlocks) + 4; }
+
 static int ensure_pr_state(ptg_graph *g) {
     if (g->d_contrib[0]) return PTG_OK;
     int rc;
     const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
     // d_part: 2 iterations x (gather partials, combine partials) + base + D0
-    const uint64_t part_floats = 2 * (2ull * g->grid + 2ull * kMaxCombineBlocks) + 4;
+    const uint64_t part_floats = pr_part_floats(g);
     if ((rc = galloc(g, (void **)&g->d_contrib[0], sizeof(float) * g->V)) ||
         (rc = galloc(g, (void **)&g->d_contrib[1], sizeof(float) * g->V)) ||
         (rc = galloc(g, (void **)&g->d_part, sizeof(float) * part_floats)) ||
@@ -304,12 +309,22 @@ void release_pr_state(ptg_graph *g) {
     g->lchunks_cap = 0;
 }
 
-int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
+// Per-iteration buffers of one PageRank run: contributions (2 x V floats),
+// dangling and base partials, hub chunk partials.
+struct PrBufs {
+    float *contrib[2];
+    float *part;
+    float *lpart;
+};
+
+
+static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st, const PrBufs &buf,
+                         bool timed) {
     int rc;
-    if ((rc = ensure_in_edges(g)) || (rc = ensure_pr_state(g))) return rc;
     const uint32_t V = g->V;
     const float r0 = 1.0f / (float)V;
     LaunchTimer timer(g);
+    if (!timed) timer.on = false;
     PTG_HIP(hipEventRecord(g->ev0, g->stream));
     if (iters == 0) {
         union {
@@ -320,12 +335,12 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
         if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(ranks), V, cv.u))) return rc;
     } else {
         const uint32_t blocks = std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid);
-        k_pr_init<<<blocks, kBlock, 0, g->stream>>>(V, g->out.off, r0, g->d_contrib[0]);
+        k_pr_init<<<blocks, kBlock, 0, g->stream>>>(V, g->out.off, r0, buf.contrib[0]);
         // partial buffers: [it parity][gather | combine], then base, then D0
         const uint32_t gb = g->grid, cb = combine_blocks(g);
-        float *part[2] = {g->d_part, g->d_part + 2 * gb};
-        float *cpart[2] = {g->d_part + 4 * gb, g->d_part + 4 * gb + 2 * kMaxCombineBlocks};
-        float *base = g->d_part + 4 * gb + 4 * kMaxCombineBlocks;
+        float *part[2] = {buf.part, buf.part + 2 * gb};
+        float *cpart[2] = {buf.part + 4 * gb, buf.part + 4 * gb + 2 * kMaxCombineBlocks};
+        float *base = buf.part + 4 * gb + 4 * kMaxCombineBlocks;
         float *d0 = base + 2;
         // D0 = (number of dangling vertices) / V, split into (hi, lo).
         const double d0v = (double)g->out.bins.n_zero / (double)V;
@@ -340,8 +355,8 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
             a.ioff = g->in.off;
             a.icol = g->in.col;
             a.ooff = g->out.off;
-            a.cin = g->d_contrib[t & 1];
-            a.cout = g->d_contrib[(t + 1) & 1];
+            a.cin = buf.contrib[t & 1];
+            a.cout = buf.contrib[(t + 1) & 1];
             a.rank_out = (t == iters - 1) ? ranks : nullptr;
             a.d0 = t == 0 ? d0 : nullptr;
             a.part_in = part[(t + 1) & 1];
@@ -349,7 +364,7 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
             a.part_out = part[t & 1];
             a.cpart_out = cpart[t & 1];
             a.base_out = base;
-            a.lpart = g->d_lpart;
+            a.lpart = buf.lpart;
             a.gather_blocks = gb;
             a.combine_blocks = cb;
             a.damping = damping;
@@ -372,6 +387,79 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
         if ((rc = timer.collect(st))) return rc;
     }
     return PTG_OK;
+}
+
+int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
+    int rc;
+    if ((rc = ensure_in_edges(g)) || (rc = ensure_pr_state(g))) return rc;
+    const PrBufs buf{{g->d_contrib[0], g->d_contrib[1]}, g->d_part, g->d_lpart};
+    return pagerank_core(g, damping, iters, ranks, st, buf, true);
+}
+
+// ---------------------------------------------------------------------------
+// PageRank without allocation: the per-iteration buffers live in a fixed
+// device array of kPrScratchBytes, one copy per GPU, shared by every call on
+// that GPU (calls that overlap on one GPU race on it). Every word a run reads
+// is written earlier in the same run (k_pr_init, the D0 pair, then each
+// iteration's partials before the next reads them), so nothing is cleared.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) uint8_t scratch[kPrScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Looks up this GPU's copy of `scratch` once, with `device` current.
+int pr_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(PTG_ERR_INVALID_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PTG_OK;
+    int cur = -1;
+    PTG_HIP(hipGetDevice(&cur));
+    if (cur != device) PTG_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = p;
+    return PTG_OK;
+}
+
+static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
+
+uint64_t pr_scratch_need(const ptg_graph *g) {
+    const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
+    return 2 * al256(4ull * g->V) + al256(4 * pr_part_floats(g)) + al256(8 * (chunks + 1));
+}
+
+int run_pagerank_noalloc(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
+    int rc;
+    // in-edges without allocating: the graph's own lists when it is
+    // symmetric, else a transpose an earlier ptg_pagerank built
+    if (!g->have_in) {
+        int sym = g->symmetric;
+        if (sym < 0) return fail(PTG_ERR_UNSUPPORTED, "pagerank_noalloc: symmetry not yet known");
+        if (!sym)
+            return fail(PTG_ERR_UNSUPPORTED,
+                        "pagerank_noalloc: graph is not symmetric and has no in-edges yet (run ptg_pagerank once)");
+        if ((rc = ensure_in_edges(g))) return rc;  // aliases the out-lists, no allocation
+    }
+    const uint64_t need = pr_scratch_need(g);
+    if (need > kPrScratchBytes)
+        return fail(PTG_ERR_BUDGET, "pagerank_noalloc: needs " + std::to_string(need) + " scratch bytes, the maximum is " +
+                                        std::to_string(kPrScratchBytes) + " (about 8M vertices)");
+    if ((rc = pr_scratch_setup(g->device))) return rc;
+    uint8_t *p = static_cast<uint8_t *>(g_scratch_ptr[g->device]);
+    PrBufs buf;
+    buf.contrib[0] = reinterpret_cast<float *>(p);
+    p += al256(4ull * g->V);
+    buf.contrib[1] = reinterpret_cast<float *>(p);
+    p += al256(4ull * g->V);
+    buf.part = reinterpret_cast<float *>(p);
+    p += al256(4 * pr_part_floats(g));
+    buf.lpart = reinterpret_cast<float *>(p);
+    return pagerank_core(g, damping, iters, ranks, st, buf, false);
 }
 
 }  // namespace ptg

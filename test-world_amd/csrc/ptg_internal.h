@@ -239,6 +239,12 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_cc(ptg_graph *g, uint32_t *out, ptg_stats *st);
 int run_pagerank(ptg_graph *g, float damping, int iters, float *out, ptg_stats *st);
+// PageRank with its per-iteration buffers in a fixed per-GPU device array of
+// kPrScratchBytes (pagerank.hip); allocates nothing.
+constexpr uint64_t kPrScratchBytes = 64ull << 20;
+int pr_scratch_setup(int device);
+uint64_t pr_scratch_need(const ptg_graph *g);
+int run_pagerank_noalloc(ptg_graph *g, float damping, int iters, float *out, ptg_stats *st);
 int check_symmetric(ptg_graph *g, int *sym);
 int ensure_in_edges(ptg_graph *g);
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,

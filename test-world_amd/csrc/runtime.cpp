@@ -459,7 +459,7 @@ int ptg_init(int device) {
     }
     if (device < 0 || device >= n) return fail(PTG_ERR_INVALID_ARG, "device index out of range");
     PTG_HIP(hipSetDevice(device));
-    return PTG_OK;
+    return pr_scratch_setup(device);
 }
 
 int ptg_device_info(int *num_cus, uint64_t *total_mem, uint64_t *free_mem) {
@@ -642,6 +642,16 @@ int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out, 
     if (g->streamed)
         return with_output(g, ranks_out, [&](float *o) { return run_pagerank_streamed(g, damping, iterations, o, st); });
     return with_output(g, ranks_out, [&](float *o) { return run_pagerank(g, damping, iterations, o, st); });
+}
+
+int ptg_pagerank_noalloc(ptg_graph *g, float damping, int iterations, float *ranks_out, ptg_stats *st) {
+    if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
+    if (iterations < 0 || !(damping >= 0.0f && damping <= 1.0f))
+        return fail(PTG_ERR_INVALID_ARG, "pagerank_noalloc: need iterations >= 0 and 0 <= damping <= 1");
+    if (g->streamed) return fail(PTG_ERR_UNSUPPORTED, "pagerank_noalloc: not available on a streamed graph");
+    if (!ranks_out || !is_device_pointer(ranks_out))
+        return fail(PTG_ERR_INVALID_ARG, "pagerank_noalloc: the output must be device memory");
+    return with_output(g, ranks_out, [&](float *o) { return run_pagerank_noalloc(g, damping, iterations, o, st); });
 }
 
 // Shared checks and output delivery of the two batched queries.

@@ -216,6 +216,7 @@ SIGNATURES = {
     "ptg_sssp": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(Stats)]),
     "ptg_cc": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Stats)]),
     "ptg_pagerank": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
+    "ptg_pagerank_noalloc": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "ptg_bfs_batch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "ptg_sssp_batch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "bfsCaculateInShare": (ctypes.c_long, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32]),
@@ -506,6 +507,11 @@ class Graph:
 
     def pagerank(self, damping=0.85, iters=20, out=None):
         return self._run(lib().ptg_pagerank, (ctypes.c_float(damping), iters), out, np.float32, "ptg_pagerank")
+
+    def pagerank_noalloc(self, out, damping=0.85, iters=20):
+        """ptg_pagerank_noalloc: no device allocation; `out` is a DeviceBuffer."""
+        return self._run(lib().ptg_pagerank_noalloc, (ctypes.c_float(damping), iters), out, np.float32,
+                         "ptg_pagerank_noalloc")
 
     def _run_batch(self, fn, srcs, out, name):
         srcs = np.ascontiguousarray(srcs, dtype=np.uint32)

@@ -67,8 +67,12 @@ enum {
 /* Keep the edges in pinned host memory, cut into chunks of at most
  * chunk_bytes, and stream them into a device cache sized by
  * device_budget_bytes (replaces the reference's partition-swap path,
- * main.cu:1681-2059). Such a graph answers BFS and PageRank (symmetric
- * graphs); other queries return PTG_ERR_UNSUPPORTED. */
+ * main.cu:1681-2059, and the Opt drivers' static prefix + overload split,
+ * main.cu:222-440, :678-889, :1180-1418). Such a graph answers BFS, SSSP
+ * (weighted graphs: the weights stream with the columns when SSSP needs
+ * them), CC (one union-find pass over the chunks on a symmetric graph, label
+ * propagation rounds otherwise) and PageRank (graphs symmetric as
+ * multigraphs); batched queries return PTG_ERR_UNSUPPORTED. */
 #define PTG_FLAG_STREAM 0x8u
 
 typedef struct ptg_config {
@@ -93,7 +97,11 @@ typedef struct ptg_config {
                                      distance >= the frontier minimum + sssp_defer_window;
                                      0 = auto (64), 0xFFFFFFFF = never */
     uint32_t sssp_defer_window;   /* 0 = auto (mean weight / 16, at least 1) */
-    uint32_t reserved;
+    uint32_t stream_pinned;       /* PTG_FLAG_STREAM: the first stream_pinned chunks stay in their
+                                     slots once loaded (at most slots - 2); in the default degree
+                                     order they hold the highest-degree vertices, like the
+                                     reference's degree-sorted static set (gpu_kernels.cu:83-96,
+                                     :420-448, main.cu:2196-2394). 0 = least recently used only */
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
@@ -184,9 +192,12 @@ PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
 /* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta,
  * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid,
- * batch_td_to_bu, batch_chunk and PTG_FLAG_TIMING. Every other field must equal
- * the graph's (PTG_ERR_INVALID_ARG otherwise): they shape allocations made at
- * creation. Results never depend on these values, only speed does. */
+ * batch_td_to_bu, batch_chunk, stream_pinned, PTG_FLAG_TIMING and, for a
+ * streamed graph, chunk_bytes (the chunks are cut again and the slot cache
+ * resized; PTG_ERR_BUDGET, with the previous size kept, when fewer than two
+ * slots would fit). Every other field must equal the graph's
+ * (PTG_ERR_INVALID_ARG otherwise): they shape allocations made at creation.
+ * Results never depend on these values, only speed does. */
 PTG_API int ptg_graph_set_config(ptg_graph *g, const ptg_config *cfg);
 PTG_API int ptg_graph_get_config(const ptg_graph *g, ptg_config *cfg);
 
@@ -232,13 +243,17 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
 /* Times the graph's own queries from `srcs` (n >= 1; the first 64 form the
  * batch) over a small grid of tuning values, one knob at a time, keeps the
  * fastest device time for each and applies the result to the graph (also
- * returned in *best). `what` is a mask of PTG_TUNE_*. Results never depend
- * on the tuning, only speed does. No reference counterpart (the reference
- * hard-codes 56 x 1024 threads and its partition fraction, main.cu:318-319). */
+ * returned in *best). `what` is a mask of PTG_TUNE_*; on a streamed graph
+ * only PTG_TUNE_STREAM applies, on a resident one every other bit
+ * (PTG_ERR_UNSUPPORTED otherwise). Results never depend on the tuning, only
+ * speed does. No reference counterpart (the reference hard-codes 56 x 1024
+ * threads, its partition fraction and fragment size, main.cu:66, :318-319). */
 #define PTG_TUNE_BFS 0x1u         /* td_to_bu, bu_to_td */
 #define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
 #define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_grid, batch_chunk */
+#define PTG_TUNE_STREAM 0x10u     /* streamed graphs only: chunk_bytes, stream_pinned, timed on BFS from
+                                     the sources plus 2 PageRank iterations (symmetric graphs) */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);
 
 /* ---- serving ----------------------------------------------------------------- */

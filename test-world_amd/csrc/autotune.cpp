@@ -1,6 +1,7 @@
 // Autotuner: picks the graph's runtime tuning (BFS push/pull switch
-// thresholds, SSSP bucket step and deferral, launch width and static chunk size of the
-// batched kernels) by timing the graph's own queries.
+// thresholds, SSSP bucket step and deferral, launch width and static chunk
+// size of the batched kernels; for a streamed graph the chunk size and the
+// number of pinned leading chunks) by timing the graph's own queries.
 //
 // Coordinate search: each knob in turn is set to every value of its grid with
 // the others held, the query that knob affects is timed twice per value (HIP
@@ -19,7 +20,7 @@ namespace ptg {
 
 namespace {
 
-enum Query { kBfs, kSssp, kBfsBatch, kSsspBatch };
+enum Query { kBfs, kSssp, kBfsBatch, kSsspBatch, kStream };
 
 struct Tuner {
     ptg_graph *g;
@@ -30,7 +31,19 @@ struct Tuner {
         ptg_stats st;
         int rc = PTG_OK;
         double t = 0;
-        if (q == kBfs || q == kSssp) {
+        if (q == kStream) {
+            // streamed graph: BFS from the sources, then 2 PageRank
+            // iterations on a symmetric graph (copies included: the query's
+            // stream waits for them)
+            for (uint32_t s : plain) {
+                if ((rc = run_bfs_streamed(g, s, g->d_val, &st))) return rc;
+                t += (double)st.kernel_ns;
+            }
+            if (g->symmetric == 1) {
+                if ((rc = run_pagerank_streamed(g, 0.85f, 2, reinterpret_cast<float *>(g->d_val), &st))) return rc;
+                t += (double)st.kernel_ns;
+            }
+        } else if (q == kBfs || q == kSssp) {
             for (uint32_t s : plain) {
                 rc = q == kBfs ? run_bfs(g, s, g->d_val, &st) : run_sssp(g, s, g->d_val, &st);
                 if (rc) return rc;
@@ -92,7 +105,41 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     if (!g || !srcs || n < 1) return fail(PTG_ERR_INVALID_ARG, "autotune needs a graph and at least one source");
     for (int i = 0; i < n; i++)
         if (srcs[i] >= g->V) return fail(PTG_ERR_INVALID_ARG, "source out of range");
+    if (g->streamed ? (what & ~PTG_TUNE_STREAM) != 0 : (what & PTG_TUNE_STREAM) != 0)
+        return fail(PTG_ERR_UNSUPPORTED, g->streamed ? "autotune: a streamed graph tunes PTG_TUNE_STREAM only"
+                                                     : "autotune: PTG_TUNE_STREAM needs a streamed graph");
     ptg_config &c = g->cfg;
+    if (g->streamed) {
+        Tuner t{g, {}, {}};
+        for (int i = 0; i < n && i < 2; i++) t.plain.push_back(g->relabeled ? g->h_perm[srcs[i]] : srcs[i]);
+        int rc = PTG_OK;
+        // chunk size: each value re-cuts the chunks (sizes the budget cannot
+        // hold two slots of are skipped)
+        double best_ns = 0;
+        uint64_t best_chunk = c.chunk_bytes;
+        if ((rc = t.time_best(kStream, &best_ns))) return rc;
+        for (uint64_t mb : {64ull, 128ull, 256ull, 512ull, 1024ull}) {
+            const uint64_t cb = mb << 20;
+            if (cb == best_chunk) continue;
+            c.chunk_bytes = cb;
+            if (stream_plan(g, cb) != PTG_OK) continue;
+            double ns = 0;
+            if ((rc = t.time_best(kStream, &ns))) break;
+            if (ns < best_ns) {
+                best_ns = ns;
+                best_chunk = cb;
+            }
+        }
+        c.chunk_bytes = best_chunk;
+        const int prc = stream_plan(g, best_chunk);
+        if (!rc) rc = prc;
+        uint32_t nsl = 0;
+        if (!rc) stream_info(g, nullptr, &nsl, nullptr);
+        const uint32_t top = nsl > 2 ? nsl - 2 : 0;
+        if (!rc) rc = t.knob(kStream, &c.stream_pinned, std::vector<uint32_t>{0, top / 2, top});
+        if (best) *best = c;
+        return rc;
+    }
     // explicit values for every knob the search moves
     if (c.td_to_bu <= 0) c.td_to_bu = 14.0f;
     if (c.bu_to_td <= 0) c.bu_to_td = 24.0f;

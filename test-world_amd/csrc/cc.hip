@@ -19,43 +19,11 @@
 
 #include "rounds.h"
 #include "static_expand.h"
+#include "union_find.h"
 
 namespace ptg {
 
 int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uint32_t *iters, uint64_t *fsum);
-
-// Root search with path halving. A parent always has a smaller key than its
-// child, so any value read, even a stale one, is an ancestor and the walk
-// terminates.
-__device__ __forceinline__ uint32_t uf_root(uint32_t *par, uint32_t x) {
-    for (;;) {
-        const uint32_t p = par[x];
-        if (p == x) return x;
-        const uint32_t gp = par[p];
-        if (gp == p) return p;
-        par[x] = gp;
-        x = gp;
-    }
-}
-
-__device__ __forceinline__ uint32_t uf_key(const uint32_t *key, uint32_t x) { return key ? key[x] : x; }
-
-// Links the two trees, always hanging the root with the larger key under the
-// other one, so every root is the minimum-key vertex of its tree.
-__device__ __forceinline__ void uf_unite(uint32_t *par, const uint32_t *key, uint32_t a, uint32_t b) {
-    uint32_t ra = uf_root(par, a), rb = uf_root(par, b);
-    while (ra != rb) {
-        if (uf_key(key, ra) < uf_key(key, rb)) {
-            const uint32_t t = ra;
-            ra = rb;
-            rb = t;
-        }
-        const uint32_t seen = atomicCAS(&par[ra], ra, rb);
-        if (seen == ra) return;
-        ra = uf_root(par, seen);
-        rb = uf_root(par, rb);
-    }
-}
 
 constexpr uint32_t kAfforestK = 2;
 constexpr uint32_t kAfforestSamples = 1024;
@@ -191,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_count_roots(uint32_t V, const uint32
     }
 }
 
-static int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots) {
+int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots) {
     unsigned long long *acc = g->d_acc;
     int rc = fill_u32(g, reinterpret_cast<uint32_t *>(acc), 2 * kShards, 0u);
     if (rc) return rc;
@@ -204,6 +172,33 @@ static int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots) {
     uint64_t c = 0;
     for (int s = 0; s < kShards; s++) c += h[s];
     *roots = (uint32_t)c;
+    return PTG_OK;
+}
+
+// par[v] = v (a union-find with every vertex its own root).
+__global__ __launch_bounds__(kBlock) void k_uf_iota(uint32_t V, uint32_t *__restrict__ par) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) par[v] = v;
+}
+
+int uf_start(ptg_graph *g, uint32_t *par) {
+    k_uf_iota<<<g->grid, kBlock, 0, g->stream>>>(g->V, par);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
+// Flattens the union-find in `par` and maps every root to its key (the
+// caller's id when the graph is relabeled): par becomes the CC labels.
+int uf_finish(ptg_graph *g, uint32_t *par) {
+    k_uf_flatten<<<g->grid, kBlock, 0, g->stream>>>(g->V, par);
+    if (g->d_inv) k_map_keys<<<g->grid, kBlock, 0, g->stream>>>(g->V, g->d_inv, par);
+    PTG_HIP(hipGetLastError());
+    return PTG_OK;
+}
+
+int init_keys(ptg_graph *g, uint32_t *val) {
+    k_init_keys<<<g->grid, kBlock, 0, g->stream>>>(g->V, g->d_inv, val);
+    PTG_HIP(hipGetLastError());
     return PTG_OK;
 }
 

@@ -12,20 +12,23 @@
 
 namespace {
 
-// `may_stream`: the query runs on a streamed graph too (BFS, PageRank on a
-// symmetric graph); when the resident graph would not fit the device's free
-// memory, such a driver streams the edges like the reference's partition
-// drivers (bfsCaculateInOpt / bfsCaculateInAsyncSwap, main.cu:1180-2059).
+// When the resident graph would not fit 90% of the device's free memory, a
+// driver streams the edges from pinned host memory like the reference's
+// partition drivers (bfsCaculateInOpt / bfsCaculateInAsyncSwap,
+// main.cu:1180-2059; ssspCaculateInOpt :678-889; ccCaculateInOpt :222-440).
+// Streamed PageRank needs a graph that is symmetric as a multigraph; on any
+// other graph that does not fit, the PageRank drivers return
+// PTG_ERR_UNSUPPORTED.
 template <class F>
 long run_one(uint32_t V, uint32_t E, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges,
              uint32_t sentinel, bool may_stream, F query) {
     ptg_config cfg;
     ptg_config_default(&cfg);
     cfg.sssp_sentinel = sentinel;
-    if (may_stream && cols) {
+    if (may_stream) {
         int cus = 0;
         uint64_t total = 0, free_b = 0;
-        const uint64_t need = ptg_graph_estimate_bytes(V, E, 0, &cfg);
+        const uint64_t need = ptg_graph_estimate_bytes(V, E, wedges ? 1 : 0, &cfg);
         if (ptg_device_info(&cus, &total, &free_b) == PTG_OK && need > free_b / 10 * 9) {
             cfg.flags |= PTG_FLAG_STREAM;
             cfg.device_budget_bytes = free_b / 10 * 9;
@@ -66,18 +69,18 @@ long bfsCaculateInAsyncSwap(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edg
 
 long ssspCaculateInShare(uint32_t V, uint32_t E, uint32_t *ptr, EdgeWithWeight *edgeList, uint32_t src) {
     // unreached = UINT_MAX - 1 (main.cu:584)
-    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFEu, false,
+    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFEu, true,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_sssp(g, src, o, nullptr); });
 }
 
 long ssspCaculateInOpt(uint32_t V, uint32_t E, uint32_t *ptr, EdgeWithWeight *edgeList, uint32_t src) {
     // unreached = UINT_MAX (main.cu:742)
-    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFFu, false,
+    return run_one(V, E, ptr, nullptr, edgeList, 0xFFFFFFFFu, true,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_sssp(g, src, o, nullptr); });
 }
 
 long ccCaculateInShare(uint32_t V, uint32_t E, uint32_t *ptr, uint32_t *edgeList) {
-    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu, false,
+    return run_one(V, E, ptr, edgeList, nullptr, 0xFFFFFFFEu, true,
                    [&](ptg_graph *g, uint32_t *o) { return ptg_cc(g, o, nullptr); });
 }
 

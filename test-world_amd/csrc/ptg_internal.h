@@ -282,7 +282,18 @@ void release_sssp_split(ptg_graph *g);
 void release_pr_state(ptg_graph *g);
 int row_checksums(ptg_graph *g, hipStream_t stream, const uint32_t *rows, uint32_t n, unsigned long long *d_out);
 // Streamed graphs (stream.hip).
-int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges);
+int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
+                    const EdgeWithWeight *wedges);
+int run_sssp_streamed(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st);
+// (Re)cuts a streamed graph into chunks of at most chunk_bytes and sizes its
+// slot cache by the budget.
+int stream_plan(ptg_graph *g, uint64_t chunk_bytes);
+int run_cc_streamed(ptg_graph *g, uint32_t *label, ptg_stats *st);
+// Union-find and label helpers of cc.hip, on the graph's stream.
+int uf_start(ptg_graph *g, uint32_t *par);
+int uf_finish(ptg_graph *g, uint32_t *par);
+int init_keys(ptg_graph *g, uint32_t *val);
+int count_roots(ptg_graph *g, const uint32_t *label, uint32_t *roots);
 void destroy_streamed(ptg_graph *g);
 int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st);
 int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st);

@@ -281,8 +281,7 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
         TRY(fail(PTG_ERR_HIP, "hipEventCreate"));
     if (streamed) {
-        g->weighted = false;  // a streamed graph keeps columns only (BFS, PageRank)
-        TRY(create_streamed(g, ptr, cols, wedges));
+        TRY(create_streamed(g, ptr, cols, weights, wedges));
         g->create_bytes = g->device_bytes;
         *out = g;
         return PTG_OK;
@@ -572,14 +571,28 @@ uint64_t ptg_graph_device_bytes(const ptg_graph *g) { return g ? g->device_bytes
 int ptg_graph_set_config(ptg_graph *g, const ptg_config *c) {
     if (!g || !c) return fail(PTG_ERR_INVALID_ARG, "null graph or config");
     const ptg_config &o = g->cfg;
-    if (c->device_budget_bytes != o.device_budget_bytes || c->chunk_bytes != o.chunk_bytes ||
+    const uint64_t o_chunk = o.chunk_bytes;
+    if (c->device_budget_bytes != o.device_budget_bytes || (c->chunk_bytes != o.chunk_bytes && !g->streamed) ||
         c->sssp_sentinel != o.sssp_sentinel || c->grid_blocks != o.grid_blocks || c->small_degree != o.small_degree ||
         c->large_degree != o.large_degree || c->order_hub_degree != o.order_hub_degree ||
         ((c->flags ^ o.flags) & ~PTG_FLAG_TIMING))
         return fail(PTG_ERR_INVALID_ARG, "only tuning fields can change after graph creation");
     if (!(c->td_to_bu >= 0) || !(c->bu_to_td >= 0) || !(c->batch_td_to_bu >= 0))
         return fail(PTG_ERR_INVALID_ARG, "negative or NaN switch threshold");
+    const bool replan = g->streamed && c->chunk_bytes != o.chunk_bytes;
     g->cfg = *c;
+    if (replan) {
+        // a streamed graph is cut into chunks of the new size; on failure
+        // (budget) the previous size is planned again
+        const uint64_t want = c->chunk_bytes;
+        int rc = stream_plan(g, want);
+        if (rc) {
+            const std::string why = ptg_last_error();
+            g->cfg.chunk_bytes = o_chunk;
+            if (stream_plan(g, o_chunk) == PTG_OK) return fail(rc, why);
+            return rc;
+        }
+    }
     return PTG_OK;
 }
 
@@ -623,15 +636,15 @@ int ptg_bfs(ptg_graph *g, uint32_t src, uint32_t *levels_out, ptg_stats *st) {
 int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
     if (src >= g->V) return fail(PTG_ERR_INVALID_ARG, "sssp: source >= V");
-    if (g->streamed) return fail(PTG_ERR_UNSUPPORTED, "sssp: not available on a streamed graph");
     if (!g->weighted) return fail(PTG_ERR_UNSUPPORTED, "sssp: graph has no weights");
     const uint32_t s = map_source(g, src);
+    if (g->streamed) return with_output(g, dist_out, [&](uint32_t *o) { return run_sssp_streamed(g, s, o, st); });
     return with_output(g, dist_out, [&](uint32_t *o) { return run_sssp(g, s, o, st); });
 }
 
 int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st) {
     if (!g) return fail(PTG_ERR_INVALID_ARG, "null graph");
-    if (g->streamed) return fail(PTG_ERR_UNSUPPORTED, "cc: not available on a streamed graph");
+    if (g->streamed) return with_output(g, labels_out, [&](uint32_t *o) { return run_cc_streamed(g, o, st); });
     return with_output(g, labels_out, [&](uint32_t *o) { return run_cc(g, o, st); });
 }
 

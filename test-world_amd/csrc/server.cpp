@@ -51,6 +51,7 @@ struct Slot {
     uint64_t last_used = 0;  // for least-recently-used release
     uint64_t base_bytes = 0; // graph bytes when created (CSR and fixed state)
     bool drained = false;    // state released since the last launch
+    bool releasing = false;  // another thread is freeing this graph's state (lock not held)
     uint32_t *row = nullptr;              // V words: single-query results (caller ids)
     unsigned long long *sums = nullptr;   // 64 row checksums (device)
     unsigned long long *h_sums = nullptr; // pinned copy
@@ -86,11 +87,29 @@ uint64_t resident(const ptg_server *s) {
     return t;
 }
 
-void release_state(Slot *sl) {
+// Frees the graph's query state; returns the bytes it still holds. Runs
+// without the server lock (hipFree waits for the device); the caller marks
+// the slot `releasing` first, so its worker does not launch meanwhile, and
+// stores the result in `held` under the lock.
+uint64_t release_state(Slot *sl) {
     destroy_batch(sl->g);
     release_pr_state(sl->g);
     release_sssp_split(sl->g);
-    sl->held = sl->g->device_bytes - sl->base_bytes;
+    return sl->g->device_bytes - sl->base_bytes;
+}
+
+// Releases `victim`'s state with the lock dropped around the frees.
+void release_unlocked(ptg_server *s, Slot *victim, std::unique_lock<std::mutex> &lk) {
+    const uint64_t before = victim->held;
+    victim->releasing = true;
+    lk.unlock();
+    const uint64_t after = release_state(victim);
+    lk.lock();
+    victim->held = after;
+    victim->releasing = false;
+    victim->drained = true;
+    if (after < before) s->evictions++;
+    s->cv_work.notify_all();
 }
 
 // Records completions of `reqs` (lock held).
@@ -206,40 +225,40 @@ void worker_loop(ptg_server *s, Slot *sl) {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         s->cv_work.wait(lk, [&] {
-            return s->stop || (!s->paused && !sl->q.empty() &&
+            return s->stop || (!s->paused && !sl->q.empty() && !sl->releasing &&
                                (!s->cfg.max_inflight || s->inflight < s->cfg.max_inflight));
         });
         if (s->stop) return;
         std::vector<Pending> job = form_launch(s, sl);
         const uint32_t kind = job.front().r.kind;
-        uint64_t need = launch_need(sl, kind, job.size());
-        // memory admission
+        uint64_t need = 0;
+        // memory admission; `need` is recomputed every time round, since
+        // another worker may have released this graph's state meanwhile
         const uint64_t budget = s->cfg.state_budget_bytes;
         bool rejected = false;
-        while (budget && resident(s) + need > budget) {
+        for (;;) {
+            if (sl->releasing) {  // another worker is freeing this graph's state
+                s->cv_work.wait(lk);
+                if (s->stop) return;
+                continue;
+            }
+            need = launch_need(sl, kind, job.size());
+            if (!budget || resident(s) + need <= budget) break;
             // idle graphs that still hold releasable state, least recently used first
             Slot *victim = nullptr;
             for (auto &p : s->slots)
-                if (p.get() != sl && !p->running && !p->drained && (!victim || p->last_used < victim->last_used))
+                if (p.get() != sl && !p->running && !p->drained && !p->releasing &&
+                    (!victim || p->last_used < victim->last_used))
                     victim = p.get();
+            if (!victim && !sl->drained) victim = sl;  // this graph's own state of other kinds
             if (victim) {
-                const uint64_t before = victim->held;
-                release_state(victim);
-                victim->drained = true;
-                if (victim->held < before) s->evictions++;
+                release_unlocked(s, victim, lk);
+                if (s->stop) return;
                 continue;
             }
-            if (!sl->drained) {  // this graph's own state of other kinds
-                const uint64_t before = sl->held;
-                release_state(sl);
-                sl->drained = true;
-                need = launch_need(sl, kind, job.size());
-                if (sl->held < before) s->evictions++;
-                continue;
-            }
-            bool others_running = false;
-            for (auto &p : s->slots) others_running = others_running || p->running;
-            if (!others_running) {
+            bool others_busy = false;
+            for (auto &p : s->slots) others_busy = others_busy || p->running || p->releasing;
+            if (!others_busy) {
                 rejected = true;
                 break;
             }

@@ -15,8 +15,14 @@
 //
 // Vertices with more than kHubEdges edges are cut into pieces of kHubEdges
 // (one wave each) so that no wave walks a hub's whole list; PageRank sums a
-// hub's pieces in a fixed order afterwards. BFS is top-down; PageRank pulls
-// over the out-lists, which equal the in-lists on a symmetric graph.
+// hub's pieces in a fixed order afterwards. BFS, SSSP and directed CC run
+// top-down frontier rounds (the reference's bfs_kernelDynamicSwap,
+// sssp_kernelOpt and cc_kernelOpt relaxations, ptGraph/gpu_kernels.cu:325-418,
+// with the same strict `<` then atomicMin rule); CC on a symmetric graph is
+// one union-find pass over every chunk; PageRank pulls over the out-lists,
+// which equal the in-lists on a symmetric graph. A weighted graph keeps its
+// weights in a second pinned array; a chunk's weights are copied into the
+// upper half of its slot only when a step needs them (SSSP).
 #include <string.h>
 
 #include <algorithm>
@@ -26,6 +32,7 @@
 #include "device_util.h"
 #include "fingerprint.h"
 #include "relabel.h"
+#include "union_find.h"
 
 namespace ptg {
 
@@ -41,7 +48,11 @@ struct ChunkDesc {
 };
 
 struct StreamState {
+    std::vector<uint32_t> h_off; // V + 1 offsets (device order), kept to re-plan chunks
     uint32_t *h_col = nullptr;  // pinned host columns
+    uint32_t *h_w = nullptr;    // pinned host weights (weighted graphs)
+    uint64_t wofs = 0;          // words from a slot's start to its weights
+    uint64_t slot_bytes = 0;
     std::vector<ChunkDesc> chunks;
     uint64_t max_chunk_edges = 0;
     // pieces (global edge sub-ranges of hub vertices) and runs (one per hub)
@@ -53,6 +64,7 @@ struct StreamState {
     uint32_t *d_cvb = nullptr, *d_cve = nullptr, *d_need = nullptr, *h_need = nullptr;
     std::vector<uint32_t *> slot;
     std::vector<int> slot_chunk;
+    std::vector<char> slot_w;   // the slot holds its chunk's weights too
     std::vector<uint64_t> slot_used;
     std::vector<int> chunk_slot;
     std::vector<hipEvent_t> slot_free, slot_ready;
@@ -103,56 +115,114 @@ struct ChunkArgs {
     uint64_t eb;
     uint32_t vh, ve;
     uint32_t pb, pe;
-    const uint32_t *slot;  // columns eb.. of the chunk
+    const uint32_t *slot;   // columns eb.. of the chunk
+    const uint32_t *wslot;  // weights eb.. of the chunk (steps that read them), else null
 };
 
-__device__ __forceinline__ void bfs_visit(uint32_t w, uint32_t *level, uint32_t *nxt, uint32_t nl, uint64_t &found) {
-    if (level[w] == 0xFFFFFFFFu) {
-        level[w] = nl;
-        const uint32_t bit = 1u << (w & 31);
-        if (!(atomicOr(&nxt[w >> 5], bit) & bit)) found++;
-    }
+__device__ __forceinline__ void mark_next(uint32_t *nxt, uint32_t v, uint64_t &found) {
+    const uint32_t bit = 1u << (v & 31);
+    if (!(atomicOr(&nxt[v >> 5], bit) & bit)) found++;
 }
 
-// Top-down expansion of the frontier vertices of one resident chunk.
-__global__ __launch_bounds__(kBlock) void k_stream_bfs(ChunkArgs c, const uint32_t *__restrict__ off,
-                                                       const uint32_t *__restrict__ pv,
-                                                       const uint32_t *__restrict__ pe0,
-                                                       const uint32_t *__restrict__ pe1, uint32_t *level,
-                                                       const uint32_t *__restrict__ cur, uint32_t *nxt, uint32_t nl,
-                                                       unsigned long long *cnt) {
+// Frontier step ops: vertex(u) loads what u's edges need once; edge() applies
+// the edge (u, v) of weight w and counts vertices newly put into the next
+// frontier.
+struct BfsStep {
+    uint32_t *level, *nxt;
+    uint32_t nl;  // level of the vertices this step reaches
+    __device__ __forceinline__ uint32_t vertex(uint32_t) const { return 0u; }
+    __device__ __forceinline__ void edge(uint32_t, uint32_t v, uint32_t, uint64_t &found) const {
+        if (level[v] == 0xFFFFFFFFu) {
+            level[v] = nl;
+            mark_next(nxt, v, found);
+        }
+    }
+};
+
+// sssp_kernel / sssp_kernelOpt relaxation (gpu_kernels.cu:152-172, :392-418):
+// new = value[u] + w in uint32; strict `<`, then atomicMin.
+struct SsspStep {
+    uint32_t *dist, *nxt;
+    __device__ __forceinline__ uint32_t vertex(uint32_t u) const { return dist[u]; }
+    __device__ __forceinline__ void edge(uint32_t du, uint32_t v, uint32_t w, uint64_t &found) const {
+        const uint32_t nd = du + w;
+        if (nd < dist[v] && nd < atomicMin(&dist[v], nd)) mark_next(nxt, v, found);
+    }
+};
+
+// cc_kernel / cc_kernelOpt (gpu_kernels.cu:136-150, :370-390): the smaller
+// label flows along the edge.
+struct LabelStep {
+    uint32_t *label, *nxt;
+    __device__ __forceinline__ uint32_t vertex(uint32_t u) const { return label[u]; }
+    __device__ __forceinline__ void edge(uint32_t lu, uint32_t v, uint32_t, uint64_t &found) const {
+        if (lu < label[v] && lu < atomicMin(&label[v], lu)) mark_next(nxt, v, found);
+    }
+};
+
+// Union-find hooking of every edge (symmetric graphs): one pass over all
+// chunks gives the components.
+struct HookStep {
+    uint32_t *par;
+    const uint32_t *key;
+    __device__ __forceinline__ uint32_t vertex(uint32_t u) const { return u; }
+    __device__ __forceinline__ void edge(uint32_t u, uint32_t v, uint32_t, uint64_t &) const { uf_unite(par, key, u, v); }
+};
+
+// One step over the frontier vertices of one resident chunk (`cur` bits;
+// every vertex when cur is null): hub pieces a wave each, other vertices 64
+// per wave step, lists above 32 edges walked by the whole wave.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_stream_step(ChunkArgs c, const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ pv,
+                                                        const uint32_t *__restrict__ pe0,
+                                                        const uint32_t *__restrict__ pe1,
+                                                        const uint32_t *__restrict__ cur, Op op,
+                                                        unsigned long long *cnt) {
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     uint64_t found = 0;
-    // hub pieces: one wave each
+    auto weight = [&](uint64_t e) -> uint32_t { return c.wslot ? c.wslot[e - c.eb] : 1u; };
     for (uint32_t i = c.pb + gw; i < c.pe; i += nw) {
         const uint32_t v = pv[i];
-        if (!((cur[v >> 5] >> (v & 31)) & 1u)) continue;
+        if (cur && !((cur[v >> 5] >> (v & 31)) & 1u)) continue;
+        const uint32_t sv = op.vertex(v);
         const uint64_t e1 = pe1[i];
-        for (uint64_t e = pe0[i] + lane; e < e1; e += 64) bfs_visit(c.slot[e - c.eb], level, nxt, nl, found);
+        for (uint64_t e = pe0[i] + lane; e < e1; e += 64) op.edge(sv, c.slot[e - c.eb], weight(e), found);
     }
-    // other vertices: 64 per wave step
     for (uint64_t base = (uint64_t)c.vh + (uint64_t)gw * 64; base < c.ve; base += (uint64_t)nw * 64) {
         const uint64_t v = base + lane;
-        const bool act = v < c.ve && ((cur[v >> 5] >> (v & 31)) & 1u);
+        const bool act = v < c.ve && (!cur || ((cur[v >> 5] >> (v & 31)) & 1u));
         uint64_t b = 0, d = 0;
+        uint32_t sv = 0;
         if (act) {
             b = off[v];
             d = off[v + 1] - b;
+            sv = op.vertex((uint32_t)v);
         }
         uint64_t mm = ballot(act && d > 32);
         while (mm) {
             const uint32_t q = lowest_lane(mm);
             mm &= mm - 1;
             const uint64_t bq = read_lane64(b, q), dq = read_lane64(d, q);
-            for (uint64_t k = lane; k < dq; k += 64) bfs_visit(c.slot[bq + k - c.eb], level, nxt, nl, found);
+            const uint32_t sq = read_lane(sv, q);
+            for (uint64_t k = lane; k < dq; k += 64) op.edge(sq, c.slot[bq + k - c.eb], weight(bq + k), found);
         }
         if (act && d <= 32)
-            for (uint64_t k = 0; k < d; k++) bfs_visit(c.slot[b + k - c.eb], level, nxt, nl, found);
+            for (uint64_t k = 0; k < d; k++) op.edge(sv, c.slot[b + k - c.eb], weight(b + k), found);
     }
     found = wave_sum_u64(found);
     if (lane == 0 && found) atomicAdd(&cnt[blockIdx.x % kShards], (unsigned long long)found);
+}
+
+// Every vertex in the frontier (bits of ids >= V stay clear).
+__global__ __launch_bounds__(kBlock) void k_stream_all(uint32_t V, uint32_t *bits) {
+    const uint32_t nwords = (V + 31) / 32;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+        const uint32_t left = V - w * 32;
+        bits[w] = left >= 32 ? 0xFFFFFFFFu : ((1u << left) - 1u);
+    }
 }
 
 // Compensated FP32 sums (TwoSum) in a fixed order.
@@ -303,6 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_pr_init(uint32_t V, const uin
     }
 }
 
+// value[src] = 1 (the reference's source value) and src in the frontier.
 __global__ void k_stream_seed(uint32_t *level, uint32_t *bits, uint32_t src) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         level[src] = 1u;
@@ -315,6 +386,8 @@ __global__ void k_stream_seed(uint32_t *level, uint32_t *bits, uint32_t src) {
 // ---------------------------------------------------------------------------
 
 static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
+
+int stream_plan(ptg_graph *g, uint64_t chunk_bytes);
 
 // Cuts the CSR into vertex-aligned chunks of at most max_edges edges (a
 // vertex with more edges than that gets chunks of its own) and the hub
@@ -409,58 +482,141 @@ static bool host_symmetric(uint32_t V, const uint32_t *off, const uint32_t *col)
     return all.f[0] == all.b[0] && all.f[1] == all.b[1];
 }
 
-int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const EdgeWithWeight *wedges) {
+int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, const uint32_t *weights,
+                    const EdgeWithWeight *wedges) {
     const uint32_t V = g->V;
     const uint64_t E = g->E;
     const ptg_config &c = g->cfg;
     int rc;
     StreamState *s = new StreamState();
     g->streamed = s;
-    // host CSR (relabeled by degree unless asked not to), columns pinned
+    const bool weighted = g->weighted;
+    // host CSR (relabeled by degree unless asked not to), columns and
+    // weights pinned
     std::vector<uint32_t> off;
     Relabeled rl;
-    const uint32_t *col = cols;
-    std::vector<uint32_t> tmp;
-    if (!cols && wedges) {
-        tmp.resize(E);
-        for (uint64_t e = 0; e < E; e++) tmp[e] = wedges[e].toNode;
-        col = tmp.data();
-    }
+    const uint32_t *col = cols, *wt = weights;
+    std::vector<uint32_t> tmp, tmpw;
     if (!(c.flags & PTG_FLAG_NO_RELABEL)) {
-        relabel(V, E, ptr, col, nullptr, nullptr, c.order_hub_degree, &rl);
+        relabel(V, E, ptr, cols, weighted ? weights : nullptr, cols ? nullptr : wedges, c.order_hub_degree, &rl);
         off.swap(rl.off);
         col = rl.col.data();
+        wt = weighted ? rl.w.data() : nullptr;
         g->relabeled = true;
     } else {
         off.resize((size_t)V + 1);
         memcpy(off.data(), ptr, 4ull * V);
         off[V] = (uint32_t)E;
+        if (!cols && wedges) {
+            tmp.resize(E);
+            if (weighted) tmpw.resize(E);
+            for (uint64_t e = 0; e < E; e++) {
+                tmp[e] = wedges[e].toNode;
+                if (weighted) tmpw[e] = wedges[e].weight;
+            }
+            col = tmp.data();
+            if (weighted) wt = tmpw.data();
+        }
     }
     if (hipHostMalloc((void **)&s->h_col, 4 * std::max<uint64_t>(E, 1), hipHostMallocDefault) != hipSuccess)
         return fail(PTG_ERR_OUT_OF_MEMORY, "pinned host columns");
     if (E) memcpy(s->h_col, col, 4 * E);
+    if (weighted) {
+        if (hipHostMalloc((void **)&s->h_w, 4 * std::max<uint64_t>(E, 1), hipHostMallocDefault) != hipSuccess)
+            return fail(PTG_ERR_OUT_OF_MEMORY, "pinned host weights");
+        if (E) memcpy(s->h_w, wt, 4 * E);
+    }
+    tmpw.clear();
+    tmpw.shrink_to_fit();
+    rl.w.clear();
+    rl.w.shrink_to_fit();
     g->symmetric = host_symmetric(V, off.data(), s->h_col) ? 1 : 0;
     tmp.clear();
     tmp.shrink_to_fit();
     rl.col.clear();
     rl.col.shrink_to_fit();
-    // chunk plan
-    s->max_chunk_edges = std::max<uint64_t>(64, (c.chunk_bytes ? c.chunk_bytes : (256ull << 20)) / 4);
+    s->h_off.swap(off);
+    s->zeros = 0;
+    for (uint32_t v = 0; v < V; v++) s->zeros += s->h_off[v + 1] == s->h_off[v];
+    // device vertex state (independent of the chunk plan)
+    const uint32_t nwords = (V + 31) / 32;
+    if ((rc = galloc(g, (void **)&g->out.off, 4ull * (V + 1))) ||
+        (rc = copy_sync(g, g->out.off, s->h_off.data(), 4ull * (V + 1), hipMemcpyHostToDevice)) ||
+        (rc = galloc(g, (void **)&s->d_bits[0], 4ull * nwords)) ||
+        (rc = galloc(g, (void **)&s->d_bits[1], 4ull * nwords)) ||
+        (rc = galloc(g, (void **)&s->d_cnt, 16 * kShards)) || (rc = galloc(g, (void **)&g->d_val, 4ull * V)) ||
+        (rc = galloc(g, (void **)&g->d_acc, 16 * kShards)) ||
+        (rc = galloc(g, (void **)&s->d_contrib[0], 4ull * V)) ||
+        (rc = galloc(g, (void **)&s->d_contrib[1], 4ull * V)) || (rc = galloc(g, (void **)&s->d_base, 16)))
+        return rc;
+    if (g->relabeled) {
+        if ((rc = galloc(g, (void **)&g->d_perm, 4ull * V)) || (rc = galloc(g, (void **)&g->d_inv, 4ull * V)) ||
+            (rc = galloc(g, (void **)&g->d_val2, 4ull * V)) ||
+            (rc = copy_sync(g, g->d_perm, rl.perm.data(), 4ull * V, hipMemcpyHostToDevice)) ||
+            (rc = copy_sync(g, g->d_inv, rl.inv.data(), 4ull * V, hipMemcpyHostToDevice)))
+            return rc;
+        g->h_perm.swap(rl.perm);
+    }
+    if (hipHostMalloc((void **)&s->h_cnt, 16 * kShards, hipHostMallocDefault) != hipSuccess)
+        return fail(PTG_ERR_OUT_OF_MEMORY, "pinned counters");
+    if (hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
+        return fail(PTG_ERR_HIP, "copy stream");
+    return stream_plan(g, c.chunk_bytes);
+}
+
+// Frees the chunk plan: its device tables, partials, slots and their events.
+static void free_plan(ptg_graph *g) {
+    StreamState *s = g->streamed;
+    if (s->copy) (void)hipStreamSynchronize(s->copy);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (auto *&p : s->slot) {
+        gfree(g, p);
+        p = nullptr;
+    }
+    for (auto e : s->slot_free)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : s->slot_ready)
+        if (e) (void)hipEventDestroy(e);
+    s->slot.clear();
+    s->slot_free.clear();
+    s->slot_ready.clear();
+    for (uint32_t **p : {&s->d_pv, &s->d_pe0, &s->d_pe1, &s->d_rv, &s->d_rpb, &s->d_rpe, &s->d_cvb, &s->d_cve,
+                         &s->d_need}) {
+        gfree(g, *p);
+        *p = nullptr;
+    }
+    gfree(g, s->d_ppart);
+    gfree(g, s->d_dpart);
+    s->d_ppart = s->d_dpart = nullptr;
+    if (s->h_need) (void)hipHostFree(s->h_need);
+    s->h_need = nullptr;
+    s->chunks.clear();
+}
+
+// Cuts the graph into chunks of at most chunk_bytes of columns and sizes the
+// slot cache by the device budget (what the vertex state leaves, at least two
+// slots, at most one per chunk; a weighted graph's slots hold the weights in
+// their upper half).
+int stream_plan(ptg_graph *g, uint64_t chunk_bytes) {
+    StreamState *s = g->streamed;
+    const uint32_t V = g->V;
+    const ptg_config &c = g->cfg;
+    int rc;
+    free_plan(g);
+    s->max_chunk_edges = std::max<uint64_t>(64, (chunk_bytes ? chunk_bytes : (256ull << 20)) / 4);
     std::vector<uint32_t> pv, pe0, pe1, rv, rpb, rpe, sv, spb, spe;
-    plan_chunks(V, off.data(), s->max_chunk_edges, &s->chunks, &pv, &pe0, &pe1, &rv, &rpb, &rpe, &sv, &spb, &spe);
+    plan_chunks(V, s->h_off.data(), s->max_chunk_edges, &s->chunks, &pv, &pe0, &pe1, &rv, &rpb, &rpe, &sv, &spb,
+                &spe);
     // split-vertex runs go after the per-chunk runs
     s->split_rb = (uint32_t)rv.size();
     rv.insert(rv.end(), sv.begin(), sv.end());
     rpb.insert(rpb.end(), spb.begin(), spb.end());
     rpe.insert(rpe.end(), spe.begin(), spe.end());
-    s->zeros = 0;
-    for (uint32_t v = 0; v < V; v++) s->zeros += off[v + 1] == off[v];
     const uint32_t C = (uint32_t)s->chunks.size();
     s->npieces = (uint32_t)pv.size();
     s->nruns = (uint32_t)rv.size();
     uint64_t largest = 1;
     for (auto &ch : s->chunks) largest = std::max<uint64_t>(largest, ch.ee - ch.eb);
-    // device vertex state
     auto upload = [&](uint32_t **d, const std::vector<uint32_t> &h) -> int {
         int r = galloc(g, (void **)d, 4 * std::max<size_t>(h.size(), 1));
         if (!r && !h.empty()) r = copy_sync(g, *d, h.data(), 4 * h.size(), hipMemcpyHostToDevice);
@@ -471,35 +627,18 @@ int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, con
         cvb[i] = s->chunks[i].vb;
         cve[i] = s->chunks[i].ve;
     }
-    if ((rc = upload(&g->out.off, off)) || (rc = upload(&s->d_pv, pv)) || (rc = upload(&s->d_pe0, pe0)) ||
-        (rc = upload(&s->d_pe1, pe1)) || (rc = upload(&s->d_rv, rv)) || (rc = upload(&s->d_rpb, rpb)) ||
-        (rc = upload(&s->d_rpe, rpe)) || (rc = upload(&s->d_cvb, cvb)) || (rc = upload(&s->d_cve, cve)))
-        return rc;
-    const uint32_t nwords = (V + 31) / 32;
-    if ((rc = galloc(g, (void **)&s->d_need, 4 * std::max(C, 1u))) ||
-        (rc = galloc(g, (void **)&s->d_bits[0], 4ull * nwords)) ||
-        (rc = galloc(g, (void **)&s->d_bits[1], 4ull * nwords)) ||
-        (rc = galloc(g, (void **)&s->d_cnt, 16 * kShards)) || (rc = galloc(g, (void **)&g->d_val, 4ull * V)) ||
-        (rc = galloc(g, (void **)&g->d_acc, 16 * kShards)) ||
-        (rc = galloc(g, (void **)&s->d_contrib[0], 4ull * V)) ||
-        (rc = galloc(g, (void **)&s->d_contrib[1], 4ull * V)) ||
+    if ((rc = upload(&s->d_pv, pv)) || (rc = upload(&s->d_pe0, pe0)) || (rc = upload(&s->d_pe1, pe1)) ||
+        (rc = upload(&s->d_rv, rv)) || (rc = upload(&s->d_rpb, rpb)) || (rc = upload(&s->d_rpe, rpe)) ||
+        (rc = upload(&s->d_cvb, cvb)) || (rc = upload(&s->d_cve, cve)) ||
+        (rc = galloc(g, (void **)&s->d_need, 4 * std::max(C, 1u))) ||
         (rc = galloc(g, (void **)&s->d_ppart, 8ull * std::max(s->npieces, 1u))) ||
-        (rc = galloc(g, (void **)&s->d_dpart, 8ull * 2 * std::max(C, 1u) * kStreamGrid)) ||
-        (rc = galloc(g, (void **)&s->d_base, 16)))
+        (rc = galloc(g, (void **)&s->d_dpart, 8ull * 2 * std::max(C, 1u) * kStreamGrid)))
         return rc;
-    if (g->relabeled) {
-        if ((rc = galloc(g, (void **)&g->d_perm, 4ull * V)) || (rc = galloc(g, (void **)&g->d_inv, 4ull * V)) ||
-            (rc = galloc(g, (void **)&g->d_val2, 4ull * V)) ||
-            (rc = copy_sync(g, g->d_perm, rl.perm.data(), 4ull * V, hipMemcpyHostToDevice)) ||
-            (rc = copy_sync(g, g->d_inv, rl.inv.data(), 4ull * V, hipMemcpyHostToDevice)))
-            return rc;
-        g->h_perm.swap(rl.perm);
-    }
-    if (hipHostMalloc((void **)&s->h_need, 4 * std::max(C, 1u), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_cnt, 16 * kShards, hipHostMallocDefault) != hipSuccess)
-        return fail(PTG_ERR_OUT_OF_MEMORY, "pinned counters");
-    // chunk slots: what the budget leaves, at least two, at most one per chunk
-    const uint64_t slot_bytes = al256(4 * largest);
+    if (hipHostMalloc((void **)&s->h_need, 4 * std::max(C, 1u), hipHostMallocDefault) != hipSuccess)
+        return fail(PTG_ERR_OUT_OF_MEMORY, "pinned chunk marks");
+    s->wofs = al256(4 * largest) / 4;
+    const uint64_t slot_bytes = al256(4 * largest) * (s->h_w ? 2 : 1);
+    s->slot_bytes = slot_bytes;
     uint64_t nslots = C;
     if (c.device_budget_bytes) {
         const uint64_t left = c.device_budget_bytes > g->device_bytes ? c.device_budget_bytes - g->device_bytes : 0;
@@ -509,6 +648,7 @@ int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, con
         return fail(PTG_ERR_BUDGET, "device budget leaves room for fewer than two chunk slots");
     s->slot.assign(nslots, nullptr);
     s->slot_chunk.assign(nslots, -1);
+    s->slot_w.assign(nslots, 0);
     s->slot_used.assign(nslots, 0);
     s->chunk_slot.assign(C, -1);
     s->slot_free.assign(nslots, nullptr);
@@ -520,58 +660,85 @@ int create_streamed(ptg_graph *g, const uint32_t *ptr, const uint32_t *cols, con
             return fail(PTG_ERR_HIP, "hipEventCreate");
         PTG_HIP(hipEventRecord(s->slot_free[i], g->stream));
     }
-    if (hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
-        return fail(PTG_ERR_HIP, "copy stream");
     return PTG_OK;
 }
 
 void destroy_streamed(ptg_graph *g) {
     StreamState *s = g->streamed;
     if (!s) return;
-    if (s->copy) {
-        (void)hipStreamSynchronize(s->copy);
-        (void)hipStreamDestroy(s->copy);
-    }
-    for (auto *p : s->slot) gfree(g, p);
-    for (auto e : s->slot_free)
-        if (e) (void)hipEventDestroy(e);
-    for (auto e : s->slot_ready)
-        if (e) (void)hipEventDestroy(e);
+    free_plan(g);
+    if (s->copy) (void)hipStreamDestroy(s->copy);
     for (auto e : s->tcopy) (void)hipEventDestroy(e);
-    for (auto *p : {s->d_pv, s->d_pe0, s->d_pe1, s->d_rv, s->d_rpb, s->d_rpe, s->d_cvb, s->d_cve, s->d_need,
-                    s->d_bits[0], s->d_bits[1]})
-        gfree(g, p);
+    for (auto *p : {s->d_bits[0], s->d_bits[1]}) gfree(g, p);
     gfree(g, s->d_cnt);
     gfree(g, s->d_contrib[0]);
     gfree(g, s->d_contrib[1]);
-    gfree(g, s->d_ppart);
-    gfree(g, s->d_dpart);
     gfree(g, s->d_base);
     if (s->h_col) (void)hipHostFree(s->h_col);
-    if (s->h_need) (void)hipHostFree(s->h_need);
+    if (s->h_w) (void)hipHostFree(s->h_w);
     if (s->h_cnt) (void)hipHostFree(s->h_cnt);
     delete s;
     g->streamed = nullptr;
 }
 
-// Makes chunk `c` resident and returns its slot; the graph's stream waits for
-// the copy. `pending[c']` marks chunks still to run in this step (kept if
-// another slot can be freed).
-static int acquire_chunk(ptg_graph *g, uint32_t c, const std::vector<char> &pending, uint32_t *slot_out) {
+// Copies `words` words from pinned host memory into a slot on the copy
+// stream, timed, after the slot's previous readers are done; the graph's
+// stream then waits for the copy.
+static int slot_copy(ptg_graph *g, uint32_t sl, uint32_t *dst, const uint32_t *src, uint64_t words) {
     StreamState *s = g->streamed;
+    const uint64_t bytes = 4 * words;
+    if (!bytes) return PTG_OK;
+    PTG_HIP(hipStreamWaitEvent(s->copy, s->slot_free[sl], 0));
+    if (s->tcopy.size() < 2ull * (s->tcopy_used + 1)) {
+        hipEvent_t a, b;
+        PTG_HIP(hipEventCreate(&a));
+        PTG_HIP(hipEventCreate(&b));
+        s->tcopy.push_back(a);
+        s->tcopy.push_back(b);
+    }
+    PTG_HIP(hipEventRecord(s->tcopy[2 * s->tcopy_used], s->copy));
+    PTG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->copy));
+    PTG_HIP(hipEventRecord(s->tcopy[2 * s->tcopy_used + 1], s->copy));
+    s->tcopy_used++;
+    PTG_HIP(hipEventRecord(s->slot_ready[sl], s->copy));
+    PTG_HIP(hipStreamWaitEvent(g->stream, s->slot_ready[sl], 0));
+    s->bytes += bytes;
+    return PTG_OK;
+}
+
+// Makes chunk `c` resident (with its weights when `need_w`) and returns its
+// slot; the graph's stream waits for the copy. `pending[c']` marks chunks
+// still to run in this step (kept if another slot can be freed).
+static int acquire_chunk(ptg_graph *g, uint32_t c, const std::vector<char> &pending, bool need_w,
+                         uint32_t *slot_out) {
+    StreamState *s = g->streamed;
+    const ChunkDesc &ch = s->chunks[c];
+    int rc;
     s->clock++;
     if (s->chunk_slot[c] >= 0) {
         const uint32_t sl = (uint32_t)s->chunk_slot[c];
         s->slot_used[sl] = s->clock;
-        s->hits++;
+        if (need_w && !s->slot_w[sl]) {  // columns resident, weights not yet
+            if ((rc = slot_copy(g, sl, s->slot[sl] + s->wofs, s->h_w + ch.eb, ch.ee - ch.eb))) return rc;
+            s->slot_w[sl] = 1;
+            s->loads++;
+        } else {
+            s->hits++;
+        }
         *slot_out = sl;
         return PTG_OK;
     }
+    // victim: an empty slot, else the least recently used one whose chunk
+    // this step no longer needs, else any; chunks below `pinned` only when
+    // nothing else is left
+    const uint64_t nsl = s->slot.size();
+    const uint32_t pinned = (uint32_t)std::min<uint64_t>(g->cfg.stream_pinned, nsl > 2 ? nsl - 2 : 0);
     int best = -1;
     uint64_t best_key = ~0ull;
     for (size_t i = 0; i < s->slot.size(); i++) {
         const int held = s->slot_chunk[i];
-        const uint64_t key = held < 0 ? 0 : (pending[held] ? (1ull << 62) + s->slot_used[i] : s->slot_used[i]);
+        uint64_t key = held < 0 ? 0 : (pending[held] ? (1ull << 62) + s->slot_used[i] : s->slot_used[i]);
+        if (held >= 0 && (uint32_t)held < pinned) key |= 1ull << 63;
         if (key < best_key) {
             best_key = key;
             best = (int)i;
@@ -582,32 +749,18 @@ static int acquire_chunk(ptg_graph *g, uint32_t c, const std::vector<char> &pend
     s->slot_chunk[sl] = (int)c;
     s->chunk_slot[c] = (int)sl;
     s->slot_used[sl] = s->clock;
-    const ChunkDesc &ch = s->chunks[c];
-    const uint64_t bytes = 4 * (ch.ee - ch.eb);
-    if (bytes) {
-        // the copy waits until the kernels that read the slot's previous chunk are done
-        PTG_HIP(hipStreamWaitEvent(s->copy, s->slot_free[sl], 0));
-        if (s->tcopy.size() < 2ull * (s->tcopy_used + 1)) {
-            hipEvent_t a, b;
-            PTG_HIP(hipEventCreate(&a));
-            PTG_HIP(hipEventCreate(&b));
-            s->tcopy.push_back(a);
-            s->tcopy.push_back(b);
-        }
-        PTG_HIP(hipEventRecord(s->tcopy[2 * s->tcopy_used], s->copy));
-        PTG_HIP(hipMemcpyAsync(s->slot[sl], s->h_col + ch.eb, bytes, hipMemcpyHostToDevice, s->copy));
-        PTG_HIP(hipEventRecord(s->tcopy[2 * s->tcopy_used + 1], s->copy));
-        s->tcopy_used++;
-        PTG_HIP(hipEventRecord(s->slot_ready[sl], s->copy));
-        PTG_HIP(hipStreamWaitEvent(g->stream, s->slot_ready[sl], 0));
-        s->bytes += bytes;
+    if ((rc = slot_copy(g, sl, s->slot[sl], s->h_col + ch.eb, ch.ee - ch.eb))) return rc;
+    s->slot_w[sl] = 0;
+    if (need_w) {
+        if ((rc = slot_copy(g, sl, s->slot[sl] + s->wofs, s->h_w + ch.eb, ch.ee - ch.eb))) return rc;
+        s->slot_w[sl] = 1;
     }
     s->loads++;
     *slot_out = sl;
     return PTG_OK;
 }
 
-static ChunkArgs chunk_args(const StreamState *s, uint32_t c, uint32_t sl) {
+static ChunkArgs chunk_args(const StreamState *s, uint32_t c, uint32_t sl, bool with_w = false) {
     const ChunkDesc &ch = s->chunks[c];
     ChunkArgs a;
     a.eb = ch.eb;
@@ -616,6 +769,7 @@ static ChunkArgs chunk_args(const StreamState *s, uint32_t c, uint32_t sl) {
     a.pb = ch.pb;
     a.pe = ch.pe;
     a.slot = s->slot[sl];
+    a.wslot = with_w ? s->slot[sl] + s->wofs : nullptr;
     return a;
 }
 
@@ -650,25 +804,24 @@ static int sum_counter(ptg_graph *g, uint64_t *v) {
     return PTG_OK;
 }
 
-int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
+// Frontier rounds over the chunks, from the frontier in d_bits[0]: each round
+// lists the chunks holding frontier vertices, expands the resident ones
+// first, then streams and expands the others (`launch(args, cur, nxt, L)`
+// enqueues one chunk's step on the graph's stream), and ends when a round
+// puts no vertex into the next frontier.
+template <class Launch>
+static int stream_rounds(ptg_graph *g, bool need_w, Launch launch, uint32_t *iters_out, uint64_t *fsum_out) {
     StreamState *s = g->streamed;
     const uint32_t V = g->V;
     const uint32_t C = (uint32_t)s->chunks.size();
     const uint32_t nwords = (V + 31) / 32;
     int rc;
-    begin_query(s);
-    PTG_HIP(hipEventRecord(g->ev0, g->stream));
-    if ((rc = fill_u32(g, level, V, 0xFFFFFFFFu)) || (rc = fill_u32(g, s->d_bits[0], nwords, 0u)) ||
-        (rc = fill_u32(g, s->d_bits[1], nwords, 0u)))
-        return rc;
-    k_stream_seed<<<1, 64, 0, g->stream>>>(level, s->d_bits[0], src);
-    PTG_HIP(hipGetLastError());
     uint32_t iters = 0;
-    uint64_t fsum = 1;
+    uint64_t fsum = 0;
     std::vector<char> pending(C, 0);
     for (uint32_t L = 0;; L++) {
         uint32_t *cur = s->d_bits[L & 1], *nxt = s->d_bits[(L + 1) & 1];
-        // which chunks hold frontier vertices
+        // which chunks hold frontier vertices (and the frontier's size)
         if ((rc = fill_u32(g, s->d_need, C, 0u)) ||
             (rc = fill_u32(g, reinterpret_cast<uint32_t *>(s->d_cnt), 4 * kShards, 0u)))
             return rc;
@@ -677,21 +830,22 @@ int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st)
                                                                    s->d_cnt);
         PTG_HIP(hipGetLastError());
         PTG_HIP(hipMemcpyAsync(s->h_need, s->d_need, 4ull * C, hipMemcpyDeviceToHost, g->stream));
-        PTG_HIP(hipStreamSynchronize(g->stream));
+        uint64_t fsize = 0;
+        if ((rc = sum_counter(g, &fsize))) return rc;  // also waits for h_need
         std::vector<uint32_t> order;
         for (uint32_t c = 0; c < C; c++)
             if (s->h_need[c] && s->chunk_slot[c] >= 0) order.push_back(c);
         for (uint32_t c = 0; c < C; c++)
             if (s->h_need[c] && s->chunk_slot[c] < 0) order.push_back(c);
         if (order.empty()) break;
+        fsum += fsize;
         for (uint32_t c : order) pending[c] = 1;
         if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(s->d_cnt), 4 * kShards, 0u))) return rc;
         for (uint32_t c : order) {
             uint32_t sl = 0;
-            if ((rc = acquire_chunk(g, c, pending, &sl))) return rc;
+            if ((rc = acquire_chunk(g, c, pending, need_w, &sl))) return rc;
             pending[c] = 0;
-            k_stream_bfs<<<kStreamGrid, kBlock, 0, g->stream>>>(chunk_args(s, c, sl), g->out.off, s->d_pv, s->d_pe0,
-                                                                s->d_pe1, level, cur, nxt, L + 2, s->d_cnt);
+            launch(chunk_args(s, c, sl, need_w), cur, nxt, L);
             PTG_HIP(hipGetLastError());
             PTG_HIP(hipEventRecord(s->slot_free[sl], g->stream));
         }
@@ -700,19 +854,121 @@ int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st)
         iters++;
         if ((rc = fill_u32(g, cur, nwords, 0u))) return rc;
         if (!found) break;
-        fsum += found;
     }
+    *iters_out = iters;
+    *fsum_out = fsum;
+    return PTG_OK;
+}
+
+// Clears both frontier bitmaps and the query's statistics.
+static int begin_rounds(ptg_graph *g) {
+    StreamState *s = g->streamed;
+    const uint32_t nwords = (g->V + 31) / 32;
+    begin_query(s);
+    int rc;
+    if ((rc = fill_u32(g, s->d_bits[0], nwords, 0u)) || (rc = fill_u32(g, s->d_bits[1], nwords, 0u))) return rc;
+    return PTG_OK;
+}
+
+static int end_query(ptg_graph *g, ptg_stats *st, uint32_t iters, uint64_t fsum) {
     PTG_HIP(hipEventRecord(g->ev1, g->stream));
     PTG_HIP(hipEventSynchronize(g->ev1));
+    if (!st) return PTG_OK;
+    memset(st, 0, sizeof *st);
+    float ms = 0;
+    PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    st->kernel_ns = (uint64_t)((double)ms * 1e6);
+    st->iterations = iters;
+    st->frontier_sum = fsum;
+    return finish_query_stats(g, st);
+}
+
+int run_bfs_streamed(ptg_graph *g, uint32_t src, uint32_t *level, ptg_stats *st) {
+    StreamState *s = g->streamed;
+    int rc;
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if ((rc = begin_rounds(g)) || (rc = fill_u32(g, level, g->V, 0xFFFFFFFFu))) return rc;
+    k_stream_seed<<<1, 64, 0, g->stream>>>(level, s->d_bits[0], src);
+    PTG_HIP(hipGetLastError());
+    uint32_t iters = 0;
+    uint64_t fsum = 0;
+    rc = stream_rounds(g, false, [&](const ChunkArgs &a, const uint32_t *cur, uint32_t *nxt, uint32_t L) {
+        k_stream_step<BfsStep><<<kStreamGrid, kBlock, 0, g->stream>>>(a, g->out.off, s->d_pv, s->d_pe0, s->d_pe1, cur,
+                                                                     BfsStep{level, nxt, L + 2}, s->d_cnt);
+    }, &iters, &fsum);
+    if (rc || (rc = end_query(g, st, iters, fsum))) return rc;
+    if (st) return reach_stats(g, level, 0xFFFFFFFFu, &st->reached, &st->edges_traversed);
+    return PTG_OK;
+}
+
+int run_sssp_streamed(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
+    StreamState *s = g->streamed;
+    if (!s->h_w) return fail(PTG_ERR_UNSUPPORTED, "sssp: graph has no weights");
+    const uint32_t sentinel = g->cfg.sssp_sentinel;
+    int rc;
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if ((rc = begin_rounds(g)) || (rc = fill_u32(g, dist, g->V, sentinel))) return rc;
+    k_stream_seed<<<1, 64, 0, g->stream>>>(dist, s->d_bits[0], src);
+    PTG_HIP(hipGetLastError());
+    uint32_t iters = 0;
+    uint64_t fsum = 0;
+    rc = stream_rounds(g, true, [&](const ChunkArgs &a, const uint32_t *cur, uint32_t *nxt, uint32_t) {
+        k_stream_step<SsspStep><<<kStreamGrid, kBlock, 0, g->stream>>>(a, g->out.off, s->d_pv, s->d_pe0, s->d_pe1, cur,
+                                                                      SsspStep{dist, nxt}, s->d_cnt);
+    }, &iters, &fsum);
+    if (rc || (rc = end_query(g, st, iters, fsum))) return rc;
+    if (st) return reach_stats(g, dist, sentinel, &st->reached, &st->edges_traversed);
+    return PTG_OK;
+}
+
+int run_cc_streamed(ptg_graph *g, uint32_t *label, ptg_stats *st) {
+    StreamState *s = g->streamed;
+    const uint32_t C = (uint32_t)s->chunks.size();
+    int rc;
+    uint32_t iters = 0;
+    uint64_t fsum = 0;
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if ((rc = begin_rounds(g))) return rc;
+    if (g->symmetric) {
+        // union-find: one pass over every chunk, resident ones first
+        if ((rc = uf_start(g, label))) return rc;
+        std::vector<uint32_t> order;
+        for (uint32_t c = 0; c < C; c++)
+            if (s->chunk_slot[c] >= 0) order.push_back(c);
+        for (uint32_t c = 0; c < C; c++)
+            if (s->chunk_slot[c] < 0) order.push_back(c);
+        std::vector<char> pending(C, 1);
+        for (uint32_t c : order) {
+            uint32_t sl = 0;
+            if ((rc = acquire_chunk(g, c, pending, false, &sl))) return rc;
+            pending[c] = 0;
+            k_stream_step<HookStep><<<kStreamGrid, kBlock, 0, g->stream>>>(
+                chunk_args(s, c, sl), g->out.off, s->d_pv, s->d_pe0, s->d_pe1, nullptr, HookStep{label, g->d_inv},
+                s->d_cnt);
+            PTG_HIP(hipGetLastError());
+            PTG_HIP(hipEventRecord(s->slot_free[sl], g->stream));
+        }
+        if ((rc = uf_finish(g, label))) return rc;
+        iters = 1;
+        fsum = g->V;
+    } else {
+        // label propagation from the caller's ids, every vertex active first
+        if ((rc = init_keys(g, label))) return rc;
+        const uint32_t nwords = (g->V + 31) / 32;
+        k_stream_all<<<std::max(1u, std::min<uint32_t>((nwords + kBlock - 1) / kBlock, g->grid)), kBlock, 0,
+                       g->stream>>>(g->V, s->d_bits[0]);
+        PTG_HIP(hipGetLastError());
+        rc = stream_rounds(g, false, [&](const ChunkArgs &a, const uint32_t *cur, uint32_t *nxt, uint32_t) {
+            k_stream_step<LabelStep><<<kStreamGrid, kBlock, 0, g->stream>>>(a, g->out.off, s->d_pv, s->d_pe0,
+                                                                           s->d_pe1, cur, LabelStep{label, nxt},
+                                                                           s->d_cnt);
+        }, &iters, &fsum);
+        if (rc) return rc;
+    }
+    if ((rc = end_query(g, st, iters, fsum))) return rc;
     if (st) {
-        memset(st, 0, sizeof *st);
-        float ms = 0;
-        PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-        st->kernel_ns = (uint64_t)((double)ms * 1e6);
-        st->iterations = iters;
-        st->frontier_sum = fsum;
-        if ((rc = finish_query_stats(g, st))) return rc;
-        if ((rc = reach_stats(g, level, 0xFFFFFFFFu, &st->reached, &st->edges_traversed))) return rc;
+        st->edges_traversed = g->E;
+        return count_roots(g, label, &st->reached);
     }
     return PTG_OK;
 }
@@ -754,7 +1010,7 @@ int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, 
             for (uint32_t i = 0; i < C; i++) {
                 const uint32_t c = (t & 1) ? C - 1 - i : i;
                 uint32_t sl = 0;
-                if ((rc = acquire_chunk(g, c, pending, &sl))) return rc;
+                if ((rc = acquire_chunk(g, c, pending, false, &sl))) return rc;
                 pending[c] = 0;
                 k_stream_pr<<<kStreamGrid, kBlock, 0, g->stream>>>(chunk_args(s, c, sl), p, s->d_pe0, s->d_pe1,
                                                                    s->d_ppart, dcur + 2ull * c * kStreamGrid);
@@ -788,11 +1044,9 @@ int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, 
 
 int stream_info(const ptg_graph *g, uint32_t *chunks, uint32_t *slots, uint64_t *slot_bytes) {
     const StreamState *s = g->streamed;
-    uint64_t largest = 1;
-    for (auto &ch : s->chunks) largest = std::max<uint64_t>(largest, ch.ee - ch.eb);
     if (chunks) *chunks = (uint32_t)s->chunks.size();
     if (slots) *slots = (uint32_t)s->slot.size();
-    if (slot_bytes) *slot_bytes = al256(4 * largest);
+    if (slot_bytes) *slot_bytes = s->slot_bytes;
     return PTG_OK;
 }
 

@@ -30,7 +30,7 @@ FLAG_NO_VALIDATE = 0x1
 FLAG_TIMING = 0x2
 FLAG_NO_RELABEL = 0x4
 FLAG_STREAM = 0x8
-TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH = 0x1, 0x2, 0x4, 0x8
+TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH, TUNE_STREAM = 0x1, 0x2, 0x4, 0x8, 0x10
 Q_BFS, Q_SSSP, Q_CC, Q_PAGERANK = 0, 1, 2, 3
 
 BFS_UNREACHED = 0xFFFFFFFF  # ptGraph/main.cu:1068
@@ -76,7 +76,7 @@ class Config(ctypes.Structure):
         ("order_hub_degree", ctypes.c_uint32),
         ("sssp_defer_degree", ctypes.c_uint32),
         ("sssp_defer_window", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("stream_pinned", ctypes.c_uint32),
     ]
 
 

@@ -1,7 +1,8 @@
-"""GPU parity of streamed graphs (PTG_FLAG_STREAM): edges in pinned host
-memory, a device chunk cache under a budget. BFS levels must equal the
-oracle's bit for bit and PageRank must stay within 1e-6 L1 of the FP64
-oracle, whatever the chunk size and however many chunks are evicted."""
+"""GPU parity of streamed graphs (PTG_FLAG_STREAM): edges (and weights) in
+pinned host memory, a device chunk cache under a budget. BFS levels, SSSP
+distances and CC labels must equal the oracle's bit for bit and PageRank must
+stay within 1e-6 L1 of the FP64 oracle, whatever the chunk size and however
+many chunks are evicted."""
 import os
 import sys
 
@@ -129,7 +130,8 @@ def test_streamed_no_relabel_and_directed():
         assert not g.is_symmetric()
         for s in (0, 99):
             np.testing.assert_array_equal(g.bfs(s)[0], oracle.bfs(dptr, dcol, s)[0])
-        for fn in (lambda: g.pagerank(0.85, 5), lambda: g.cc(), lambda: g.bfs_batch(np.array([0, 1], np.uint32))):
+        np.testing.assert_array_equal(g.cc()[0], oracle.cc_reach_min(dptr, dcol))  # label propagation rounds
+        for fn in (lambda: g.pagerank(0.85, 5), lambda: g.bfs_batch(np.array([0, 1], np.uint32))):
             with pytest.raises(pg.PtgError) as e:
                 fn()
             assert e.value.code == pg.ERR_UNSUPPORTED
@@ -145,3 +147,124 @@ def test_streamed_budget_and_probe():
     gbs = pg.host_link_probe(64 << 20, 4)
     say("case host link %.1f GB/s" % gbs)
     assert gbs > 1.0
+
+
+def weighted_case(case):
+    if case == "split_hub":
+        rng = np.random.default_rng(2)
+        ptr, col = star_plus(rng, 6000, 3000, 20000)
+        E = col.size
+        nxt = np.append(ptr[1:], E).astype(np.int64)
+        src = np.repeat(np.arange(ptr.size), nxt - ptr.astype(np.int64))
+        lo, hi = np.minimum(src, col), np.maximum(src, col)
+        w = ((lo * 2654435761 + hi * 40503) % 64 + 1).astype(np.uint32)  # symmetric weights in [1, 64]
+        return ptr, col, w
+    csr = pg.gen_rmat(12, 16, seed=1).add_weights(1, 64, seed=2)
+    return csr.node_pointers, csr.cols, csr.weights
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_streamed_sssp_and_cc(case):
+    chunk, small = CASES[case]
+    ptr, col, w = weighted_case(case)
+    say("case streamed sssp/cc %s V=%d E=%d" % (case, ptr.size, col.size))
+    cfg = pg.default_config(flags=pg.FLAG_STREAM, chunk_bytes=chunk)
+    if small:
+        with pg.Graph(ptr, col, w, config=cfg) as g:
+            nch, nslots, slot = g.stream_info()
+            cfg.device_budget_bytes = g.device_bytes - nslots * slot + 2 * slot
+    with pg.Graph(ptr, col, w, config=cfg) as g:
+        nch, nslots, _ = g.stream_info()
+        if small:
+            assert nslots == 2 and nch > 2
+        deg = np.diff(np.append(ptr.astype(np.int64), col.size))
+        for s in (0, int(np.argmax(deg)), ptr.size - 1):
+            d, st = g.sssp(s)
+            want = oracle.dijkstra(ptr, col, w, s)
+            np.testing.assert_array_equal(d, want, err_msg="sssp src %d" % s)
+            reached = want != oracle.SSSP_SENTINEL_SHARE
+            assert st.reached == int(reached.sum())
+            assert st.edges_traversed == int(deg[reached].sum())
+            np.testing.assert_array_equal(g.bfs(s)[0], oracle.bfs(ptr, col, s)[0])  # columns only
+        lab, st = g.cc()
+        want = oracle.cc_union_find(ptr, col)
+        np.testing.assert_array_equal(lab, want)
+        assert st.reached == int(np.unique(want).size)
+        r, _ = g.pagerank(0.85, 10)
+        assert float(np.abs(r.astype(np.float64) - oracle.pagerank_f64(ptr, col, 0.85, 10)).sum()) <= PR_L1
+
+
+def test_streamed_aos_opt_sentinel_and_no_relabel():
+    ptr, col, w = weighted_case("small_chunks")
+    aos = np.stack([col, w], axis=1)
+    say("case streamed aos opt sentinel")
+    for flags in (pg.FLAG_STREAM, pg.FLAG_STREAM | pg.FLAG_NO_RELABEL):
+        cfg = pg.default_config(flags=flags, chunk_bytes=16 << 10, sssp_sentinel=oracle.SSSP_SENTINEL_OPT)
+        with pg.Graph.from_aos(ptr, aos, config=cfg) as g:
+            for s in (0, 5, 4000):
+                np.testing.assert_array_equal(g.sssp(s)[0], oracle.dijkstra(ptr, col, w, s, oracle.SSSP_SENTINEL_OPT))
+            np.testing.assert_array_equal(g.cc()[0], oracle.cc_union_find(ptr, col))
+
+
+def test_streamed_directed_sssp():
+    rng = np.random.default_rng(11)
+    V, E = 4000, 20000
+    a, b = rng.integers(0, V, E), rng.integers(0, V, E)
+    o = np.lexsort((b, a))
+    a, b = a[o], b[o]
+    dptr = np.searchsorted(a, np.arange(V)).astype(np.uint32)
+    dcol = b.astype(np.uint32)
+    dw = rng.integers(0, 9, E).astype(np.uint32)  # zero weights included
+    say("case streamed directed sssp")
+    cfg = pg.default_config(flags=pg.FLAG_STREAM, chunk_bytes=4 << 10)
+    with pg.Graph(dptr, dcol, dw, config=cfg) as g:
+        cfg.device_budget_bytes = g.device_bytes - g.stream_info()[1] * g.stream_info()[2] + 2 * g.stream_info()[2]
+    with pg.Graph(dptr, dcol, dw, config=cfg) as g:
+        assert g.stream_info()[1] == 2
+        for s in (0, 17, V - 1):
+            np.testing.assert_array_equal(g.sssp(s)[0], oracle.dijkstra(dptr, dcol, dw, s))
+        np.testing.assert_array_equal(g.cc()[0], oracle.cc_reach_min(dptr, dcol))
+
+
+def test_streamed_replan_pinned_and_autotune():
+    """Re-cutting the chunks (set_config chunk_bytes), pinning the leading
+    chunks and the streamed autotuner change no result; a chunk size the
+    budget cannot hold two slots of is refused and the old plan kept."""
+    ptr, col, w = weighted_case("small_chunks")
+    say("case streamed replan")
+    cfg = pg.default_config(flags=pg.FLAG_STREAM, chunk_bytes=16 << 10)
+    with pg.Graph(ptr, col, w, config=cfg) as g:
+        nch, nslots, slot = g.stream_info()
+        cfg.device_budget_bytes = g.device_bytes - nslots * slot + 4 * slot  # four slots
+    want_b = oracle.bfs(ptr, col, 0)[0]
+    want_d = oracle.dijkstra(ptr, col, w, 5)
+    want_r = oracle.pagerank_f64(ptr, col, 0.85, 10)
+    with pg.Graph(ptr, col, w, config=cfg) as g:
+        nch0, nsl0, _ = g.stream_info()
+        assert nsl0 == 4 and nch0 > 4
+
+        def check():
+            np.testing.assert_array_equal(g.bfs(0)[0], want_b)
+            np.testing.assert_array_equal(g.sssp(5)[0], want_d)
+            r, _ = g.pagerank(0.85, 10)
+            assert float(np.abs(r.astype(np.float64) - want_r).sum()) <= PR_L1
+
+        check()
+        g.tune(chunk_bytes=8 << 10)  # smaller chunks: more of them, more slots in the same budget
+        nch1, nsl1, _ = g.stream_info()
+        assert nch1 > nch0 and nsl1 >= nsl0
+        check()
+        g.tune(stream_pinned=2)
+        _, st = g.pagerank(0.85, 3)
+        check()
+        with pytest.raises(pg.PtgError) as e:
+            g.tune(chunk_bytes=1 << 30)  # one chunk of every edge: a weighted slot pair no longer fits
+        assert e.value.code == pg.ERR_BUDGET
+        assert g.stream_info()[:2] == (nch1, nsl1) and g.get_config().chunk_bytes == 8 << 10
+        check()
+        c = g.autotune(np.array([0, 5], np.uint32), what=pg.TUNE_STREAM)
+        assert c.chunk_bytes in [x << 20 for x in (64, 128, 256, 512, 1024)] + [8 << 10]
+        check()
+        with pytest.raises(pg.PtgError) as e:
+            g.autotune(np.array([0], np.uint32), what=pg.TUNE_BFS)
+        assert e.value.code == pg.ERR_UNSUPPORTED

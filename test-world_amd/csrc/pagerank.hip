@@ -117,7 +117,7 @@ __device__ __forceinline__ float pr_base(const PrArgs &a, CF *lds) {
     return (1.0f - a.damping) * a.inv_v + a.damping * (dang * a.inv_v);
 }
 
-constexpr uint32_t kPrHubLds = 4096;
+constexpr uint32_t kPrHubLds = 5088;  // 20352 B: 8 workgroups of the grid per CU still fit
 
 // Gathers every vertex's in-neighbour contributions. Hub lists are cut into
 // chunks whose partial sums go to lpart; k_pr_combine finishes those
@@ -125,9 +125,10 @@ constexpr uint32_t kPrHubLds = 4096;
 __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
     __shared__ CF lds[kWavesPerBlock];
     // The contributions of the first kPrHubLds vertices are staged in LDS:
-    // in the default degree order these are the hubs, which are 22% of the
-    // in-list entries at R-MAT scale 22 (16 KB leaves the 8 workgroups per
-    // CU of the grid; 32 KB lowered occupancy and was 11% slower).
+    // in the default degree order these are the hubs, about a quarter of the
+    // in-list entries at R-MAT scale 22. 20 KB leaves room for the 8
+    // workgroups per CU of the grid; 32 KB (5 per CU) was 11% slower, and
+    // 128 KB in 1024-thread workgroups (1 per CU) 20% slower.
     __shared__ float s_hub[kPrHubLds];
     const uint32_t nh = min(kPrHubLds, a.n_vert);
     for (uint32_t i = threadIdx.x; i < nh; i += kBlock) s_hub[i] = a.cin[i];

@@ -88,7 +88,8 @@ typedef struct ptg_config {
                                      light); 0 = one unbounded bucket, no split */
     uint32_t flags;               /* PTG_FLAG_* */
     uint32_t batch_delta;         /* near/far step of batched SSSP; 0 = auto */
-    uint32_t batch_grid;          /* workgroups of the batched kernels; 0 = 4 x grid_blocks */
+    uint32_t batch_grid;          /* workgroups of the batched BFS (and, unless batch_sssp_grid
+                                     is set, SSSP) kernels; 0 = 4 x grid_blocks */
     float batch_td_to_bu;         /* batched BFS goes bottom-up when m_f > m_u / this; 0 = td_to_bu */
     uint32_t batch_chunk;         /* edges per static chunk of batched SSSP; 0 = auto (128) */
     uint32_t order_hub_degree;    /* vertex order on the device: 0 = by descending degree; d > 0 =
@@ -102,6 +103,8 @@ typedef struct ptg_config {
                                      order they hold the highest-degree vertices, like the
                                      reference's degree-sorted static set (gpu_kernels.cu:83-96,
                                      :420-448, main.cu:2196-2394). 0 = least recently used only */
+    uint32_t batch_sssp_grid;     /* workgroups of the batched SSSP kernels; 0 = batch_grid */
+    uint32_t reserved;
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
@@ -191,7 +194,7 @@ PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32
  * reference counterpart: the reference times whole calls with the host clock. */
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
 /* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta,
- * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid,
+ * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid, batch_sssp_grid,
  * batch_td_to_bu, batch_chunk, stream_pinned, PTG_FLAG_TIMING and, for a
  * streamed graph, chunk_bytes (the chunks are cut again and the slot cache
  * resized; PTG_ERR_BUDGET, with the previous size kept, when fewer than two
@@ -251,7 +254,7 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
 #define PTG_TUNE_BFS 0x1u         /* td_to_bu, bu_to_td */
 #define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
-#define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_grid, batch_chunk */
+#define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_sssp_grid, batch_chunk */
 #define PTG_TUNE_STREAM 0x10u     /* streamed graphs only: chunk_bytes, stream_pinned, timed on BFS from
                                      the sources plus 2 PageRank iterations (symmetric graphs) */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);

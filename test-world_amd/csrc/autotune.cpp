@@ -145,6 +145,7 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     if (c.bu_to_td <= 0) c.bu_to_td = 24.0f;
     if (c.batch_td_to_bu <= 0) c.batch_td_to_bu = c.td_to_bu;
     if (!c.batch_grid) c.batch_grid = 4 * g->grid;
+    if (!c.batch_sssp_grid) c.batch_sssp_grid = c.batch_grid;
     if (!c.batch_delta) c.batch_delta = 256;
     if (!c.batch_chunk) c.batch_chunk = 128;
     Tuner t{g, {}, {}};
@@ -165,11 +166,11 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     }
     if (what & PTG_TUNE_BFS_BATCH) {
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_td_to_bu, std::vector<float>{2, 6, 14, 30});
-        if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G});
+        if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G, 8 * G});
     }
     if ((what & PTG_TUNE_SSSP_BATCH) && g->weighted) {
-        if (!rc) rc = t.knob(kSsspBatch, &c.batch_delta, std::vector<uint32_t>{32, 64, 128, 256, 1024});
-        if (!rc) rc = t.knob(kSsspBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G});
+        if (!rc) rc = t.knob(kSsspBatch, &c.batch_delta, std::vector<uint32_t>{32, 64, 128, 256, 1024, 4096});
+        if (!rc) rc = t.knob(kSsspBatch, &c.batch_sssp_grid, std::vector<uint32_t>{G, 2 * G, 4 * G, 8 * G, 16 * G, 32 * G, 64 * G});
         if (!rc) rc = t.knob(kSsspBatch, &c.batch_chunk, std::vector<uint32_t>{64, 128, 256, 512, 1024});
     }
     if (best) *best = c;

@@ -682,7 +682,7 @@ static int ensure_batch(ptg_graph *g, bool sssp) {
     return PTG_OK;
 }
 
-static BatchEnv make_batch_env(ptg_graph *g, int sym) {
+static BatchEnv make_batch_env(ptg_graph *g, int sym, bool sssp) {
     BatchState *b = g->batch;
     BatchEnv e{};
     e.V = g->V;
@@ -709,6 +709,7 @@ static BatchEnv make_batch_env(ptg_graph *g, int sym) {
     e.delta = g->cfg.batch_delta ? g->cfg.batch_delta : 256u;
     e.symmetric = sym;
     e.grid = g->cfg.batch_grid ? g->cfg.batch_grid : 4 * g->grid;
+    if (sssp && g->cfg.batch_sssp_grid) e.grid = g->cfg.batch_sssp_grid;
     return e;
 }
 
@@ -767,7 +768,7 @@ static int run_batch(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, 
     if ((rc = check_symmetric(g, &sym))) return rc;
     if ((rc = ensure_batch(g, sssp))) return rc;
     BatchState *b = g->batch;
-    const BatchEnv env = make_batch_env(g, sym);
+    const BatchEnv env = make_batch_env(g, sym, sssp);
     hipGraphExec_t &exec = sssp ? b->sssp_exec : b->bfs_exec;
     BatchEnv &cap_env = sssp ? b->env_sssp : b->env_bfs;
     if (exec && !same_env(env, cap_env)) {

@@ -77,6 +77,8 @@ class Config(ctypes.Structure):
         ("sssp_defer_degree", ctypes.c_uint32),
         ("sssp_defer_window", ctypes.c_uint32),
         ("stream_pinned", ctypes.c_uint32),
+        ("batch_sssp_grid", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -197,7 +197,8 @@ def test_autotune_and_set_config():
         want_d, _ = g.sssp_batch(srcs)
         c = g.autotune(srcs)
         got = g.get_config()
-        for f in ("td_to_bu", "bu_to_td", "batch_td_to_bu", "batch_grid", "batch_delta", "batch_chunk"):
+        for f in ("td_to_bu", "bu_to_td", "batch_td_to_bu", "batch_grid", "batch_sssp_grid", "batch_delta",
+                  "batch_chunk"):
             assert getattr(c, f) == getattr(got, f) and getattr(c, f) > 0, f
         for f in ("sssp_delta", "sssp_defer_degree", "sssp_defer_window"):  # 0 = automatic
             assert getattr(c, f) == getattr(got, f), f
@@ -209,7 +210,7 @@ def test_autotune_and_set_config():
             np.testing.assert_array_equal(g.bfs(int(srcs[k]))[0], want_b[k])
             np.testing.assert_array_equal(g.sssp(int(srcs[k]))[0], want_d[k])
         # every tuning knob accepted; structural fields refused
-        g.tune(batch_chunk=32, batch_delta=7, batch_grid=3, sssp_delta=5, td_to_bu=1.5)
+        g.tune(batch_chunk=32, batch_delta=7, batch_grid=3, batch_sssp_grid=5, sssp_delta=5, td_to_bu=1.5)
         np.testing.assert_array_equal(g.sssp_batch(srcs)[0], want_d)
         np.testing.assert_array_equal(g.bfs_batch(srcs)[0], want_b)
         np.testing.assert_array_equal(g.sssp(int(srcs[3]))[0], want_d[3])

@@ -64,15 +64,13 @@ def measured_traffic(kernel, scale):
     that summary does not cover this kernel and workload."""
     if scale != 22 or not os.path.exists(TRAFFIC_FILE):
         return None
-    meta = {}
     with open(TRAFFIC_FILE) as fh:
         for line in fh:
-            if ":" in line and line.startswith("traffic_"):
-                k, v = line.split(":", 1)
-                meta[k.strip()] = v.strip()
-    if meta.get("traffic_kernel") != kernel:
-        return None
-    return int(meta["traffic_bytes_per_launch"])
+            if line.startswith("traffic_per_launch:"):
+                name, _, val = line.split(":", 1)[1].rpartition("=")
+                if name.strip() == kernel:
+                    return int(val.strip())
+    return None
 
 
 def host_info():

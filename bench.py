@@ -38,8 +38,9 @@ After the timed steps (none of this is in `value`):
            partition for that budget).
   serving  6 resident R-MAT graphs (scales 18..24), 3000 requests, mix 50%
            BFS / 30% SSSP / 15% CC / 5% PageRank(10), Poisson arrivals at
-           700/s (seed 4, near the plain path's saturation) under a 16 GiB
-           query-state budget that forces evictions: the plain per-query path
+           700/s (seed 4, near the plain path's saturation) under a 24 GiB
+           query-state budget (the graphs' batched state alone would take
+           about 27 GB, so the budget forces evictions): the plain per-query path
            (one at a time) vs the server (continuous batching); p50/p99,
            mean batch and evictions at that load, and queries/s with every
            request queued at once; checksums must match.
@@ -369,7 +370,7 @@ def main():
     ap.add_argument("--partition-budget-gb", type=float, default=4.0)
     ap.add_argument("--serve-scales", default="18,20,21,22,23,24")
     ap.add_argument("--serve-qps", type=float, default=700.0)
-    ap.add_argument("--serve-budget-gb", type=float, default=16.0,
+    ap.add_argument("--serve-budget-gb", type=float, default=24.0,
                     help="query-state budget of both servers (0 = unlimited)")
     ap.add_argument("--batch-graphs", default="rmat24,er24",
                     help="graphs of the batched leg at the stated size (empty: skip)")

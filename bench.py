@@ -339,6 +339,36 @@ def partition_leg(args, pg, np, cores, log):
                              "link_gbs": round(st.host_link_bytes / max(st.host_link_ns, 1), 2),
                              "link_vs_lower_bound": round(st.host_link_bytes / max(lower, 1), 3),
                              "chunk_loads": st.chunk_loads, "chunk_hits": st.chunk_hits}
+        t = time.perf_counter()
+        _, st = g.cc(out=out)
+        dt = time.perf_counter() - t
+        res["cc"] = {"ms": round(dt * 1e3, 1), "components": st.reached, "link_bytes": st.host_link_bytes,
+                     "link_gbs": round(st.host_link_bytes / max(st.host_link_ns, 1), 2)}
+
+        def bfs_pr(tag):
+            t = time.perf_counter()
+            _, sb = g.bfs(0, out=out)
+            tb = time.perf_counter() - t
+            t = time.perf_counter()
+            _, sp = g.pagerank(0.85, 10, out=out)
+            tp = time.perf_counter() - t
+            r = {"bfs_ms": round(tb * 1e3, 1), "bfs_link_bytes": sb.host_link_bytes,
+                 "pagerank10_ms": round(tp * 1e3, 1), "pagerank10_link_bytes": sp.host_link_bytes}
+            log("partition %s: %s" % (tag, r))
+            return r
+
+        # degree-aware residency: the leading chunks (the highest-degree
+        # vertices) pinned in their slots, against least-recently-used only
+        g.tune(stream_pinned=max(0, slots - 2))
+        bfs_pr("warm")
+        res["pinned"] = dict(stream_pinned=max(0, slots - 2), **bfs_pr("pinned"))
+        g.tune(stream_pinned=0)
+        if not args.no_tune:
+            t = time.time()
+            c = g.autotune(np.array([0], np.uint32), what=pg.TUNE_STREAM)
+            res["tuned"] = dict(chunk_bytes=c.chunk_bytes, stream_pinned=c.stream_pinned,
+                                chunks=g.stream_info()[0], slots=g.stream_info()[1],
+                                tune_s=round(time.time() - t, 1), **bfs_pr("tuned"))
     finally:
         out.free()
         g.close()

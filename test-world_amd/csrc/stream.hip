@@ -773,6 +773,13 @@ static ChunkArgs chunk_args(const StreamState *s, uint32_t c, uint32_t sl, bool 
     return a;
 }
 
+// A graph whose last re-plan failed twice has no slots; its queries are refused.
+static int check_plan(const ptg_graph *g) {
+    const StreamState *s = g->streamed;
+    if (s->slot.empty() || s->chunks.empty()) return fail(PTG_ERR_BUDGET, "streamed graph has no chunk plan");
+    return PTG_OK;
+}
+
 static void begin_query(StreamState *s) {
     s->tcopy_used = 0;
     s->loads = s->hits = s->bytes = 0;
@@ -864,8 +871,9 @@ static int stream_rounds(ptg_graph *g, bool need_w, Launch launch, uint32_t *ite
 static int begin_rounds(ptg_graph *g) {
     StreamState *s = g->streamed;
     const uint32_t nwords = (g->V + 31) / 32;
-    begin_query(s);
     int rc;
+    if ((rc = check_plan(g))) return rc;
+    begin_query(s);
     if ((rc = fill_u32(g, s->d_bits[0], nwords, 0u)) || (rc = fill_u32(g, s->d_bits[1], nwords, 0u))) return rc;
     return PTG_OK;
 }
@@ -976,6 +984,7 @@ int run_cc_streamed(ptg_graph *g, uint32_t *label, ptg_stats *st) {
 int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
     StreamState *s = g->streamed;
     if (!g->symmetric) return fail(PTG_ERR_UNSUPPORTED, "streamed PageRank needs a symmetric graph");
+    if (int prc = check_plan(g)) return prc;
     const uint32_t V = g->V;
     const uint32_t C = (uint32_t)s->chunks.size();
     const float r0 = 1.0f / (float)V;

@@ -555,8 +555,9 @@ def main():
 
     per_kind = kind_stats(acc)
     per_kind_warm = kind_stats(wacc)
-    kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false>", "cc": "k_expand<HookRest>",
-                    "pagerank": "k_pr_gather"}
+    # SSSP: a round is the push kernel plus, behind it, the pull kernel
+    kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false> + k_pull_round",
+                    "cc": "k_expand<HookRest>", "pagerank": "k_pr_gather"}
     achieved = per_kind[dom]["achieved_gbs"]
     roofline = None if args.no_timing else {
         "bound": "hbm",

@@ -104,7 +104,12 @@ typedef struct ptg_config {
                                      reference's degree-sorted static set (gpu_kernels.cu:83-96,
                                      :420-448, main.cu:2196-2394). 0 = least recently used only */
     uint32_t batch_sssp_grid;     /* workgroups of the batched SSSP kernels; 0 = batch_grid */
-    uint32_t reserved;
+    float sssp_pull;              /* SSSP on a symmetric graph, one unbounded bucket: a large round
+                                     pulls (every vertex that can still improve takes the minimum
+                                     over its in-edges) when those vertices' in-edges x sssp_pull
+                                     < the frontier's out-edges; the distances are the same.
+                                     0 = auto (1.0), < 0 = never (the in-edge pairs, 8 bytes per
+                                     edge, are then not built) */
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
@@ -118,11 +123,12 @@ typedef struct ptg_stats {
     uint64_t host_link_bytes; /* bytes copied host -> device during the query */
     uint64_t host_link_ns;    /* time of those copies */
     uint64_t main_kernel_ns;  /* PTG_FLAG_TIMING: summed HIP-event time of the query's
-                                 main kernel (frontier expansion / PageRank sweep) */
-    uint32_t main_launches;   /* PTG_FLAG_TIMING: launches of that kernel */
+                                 main kernel (frontier expansion / PageRank sweep; SSSP:
+                                 the push and the pull kernel of every round) */
+    uint32_t main_launches;   /* PTG_FLAG_TIMING: launches of that kernel (SSSP: rounds) */
     uint32_t chunk_loads;     /* PTG_FLAG_STREAM: chunks copied from the host during the query */
     uint32_t chunk_hits;      /* PTG_FLAG_STREAM: chunk uses served from the device cache */
-    uint32_t pad;
+    uint32_t pull_rounds;     /* SSSP: rounds that pulled over in-edges (sssp_pull) */
 } ptg_stats;
 
 typedef struct ptg_graph ptg_graph;

@@ -162,7 +162,10 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kSssp, &c.sssp_delta, std::vector<uint32_t>{0, 24, 64, 128});
         if (!rc) rc = t.knob(kSssp, &c.sssp_defer_window, std::vector<uint32_t>{0, 1, 3, 4, 6});
         if (!rc) rc = t.knob(kSssp, &c.sssp_defer_degree, std::vector<uint32_t>{0, 32, 128, 256, 0xFFFFFFFFu});
+        // 0 = pull when the open in-edges are fewer than the frontier's edges; -1 = never
+        if (!rc) rc = t.knob(kSssp, &c.sssp_pull, std::vector<float>{0, 0.5f, 2, -1});
         if (!rc && !c.sssp_delta) release_sssp_split(g);  // the light/heavy CSRs tried above
+        if (!rc && c.sssp_pull < 0) release_sssp_pull(g);
     }
     if (what & PTG_TUNE_BFS_BATCH) {
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_td_to_bu, std::vector<float>{2, 6, 14, 30});

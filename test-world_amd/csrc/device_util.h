@@ -30,6 +30,14 @@ __device__ __forceinline__ uint64_t read_lane64(uint64_t x, uint32_t lane) {
     return (uint64_t)hi << 32 | lo;
 }
 
+// Orders the wave's LDS accesses: writes before the call are visible to
+// reads after it by any lane of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave-wide sums / max; every lane of the wave must be active.
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     for (int d = 32; d >= 1; d >>= 1) {
@@ -38,6 +46,8 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     }
     return x;
 }
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) { return ~wave_max_u32(~x); }
 
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
     for (int d = 32; d >= 1; d >>= 1) {

@@ -10,6 +10,10 @@
 // at most E / p (Schwartz-Zippel): 2^-30 at E = 2^31. Two independent key
 // pairs are used, so a non-symmetric graph passes with probability at most
 // (E / p)^2. A symmetric graph always passes, with any keys.
+//
+// Weights are symmetric when the multiset of triples (v, u, w) equals that of
+// (u, v, w); the factors are then (r - v - s*u - t*w), linear in (r, s, t)
+// and distinct for distinct triples, so the same bound holds.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,9 +47,16 @@ __host__ __device__ __forceinline__ uint64_t fp_term(uint64_t r, uint64_t s, uin
     return r >= t ? r - t : r + kFpP - t;
 }
 
-// Two key pairs; accumulated products forward (v, u) and backward (u, v).
+// r - v - s*u - t*w mod p.
+__host__ __device__ __forceinline__ uint64_t fp_term_w(uint64_t r, uint64_t s, uint64_t t, uint32_t v, uint32_t u,
+                                                       uint32_t w) {
+    const uint64_t x = fp_reduce(fp_reduce(fp_mul(s, u) + fp_mul(t, w)) + v);  // each sum < 2^62
+    return r >= x ? r - x : r + kFpP - x;
+}
+
+// Two key sets; accumulated products forward (v, u) and backward (u, v).
 struct FpKeys {
-    uint64_t r[2], s[2];
+    uint64_t r[2], s[2], t[2];
 };
 
 struct FpAcc {
@@ -57,6 +68,14 @@ __host__ __device__ __forceinline__ void fp_edge(const FpKeys &k, FpAcc &a, uint
     for (int i = 0; i < 2; i++) {
         a.f[i] = fp_mul(a.f[i], fp_term(k.r[i], k.s[i], v, u));
         a.b[i] = fp_mul(a.b[i], fp_term(k.r[i], k.s[i], u, v));
+    }
+}
+
+__host__ __device__ __forceinline__ void fp_edge_w(const FpKeys &k, FpAcc &a, uint32_t v, uint32_t u, uint32_t w) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        a.f[i] = fp_mul(a.f[i], fp_term_w(k.r[i], k.s[i], k.t[i], v, u, w));
+        a.b[i] = fp_mul(a.b[i], fp_term_w(k.r[i], k.s[i], k.t[i], u, v, w));
     }
 }
 

@@ -123,6 +123,20 @@ struct FpOp {
     }
 };
 
+// Triples (v, u, w) from (column, weight) pairs.
+struct FpWOp {
+    const uint2 *__restrict__ cw;
+    FpKeys k;
+    mutable FpAcc a;
+    __device__ __forceinline__ uint32_t vertex(uint32_t v) const { return v; }
+    __device__ __forceinline__ void edge(bool valid, uint32_t v, uint64_t e) const {
+        if (valid) {
+            const uint2 x = cw[e];
+            fp_edge_w(k, a, v, x.x, x.y);
+        }
+    }
+};
+
 __device__ __forceinline__ uint64_t wave_fp_product(uint64_t x) {
     for (int d = 32; d >= 1; d >>= 1) {
         const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, d, 64);
@@ -132,11 +146,10 @@ __device__ __forceinline__ uint64_t wave_fp_product(uint64_t x) {
     return x;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fingerprint(Frontier in, const uint32_t *__restrict__ off,
-                                                        const uint32_t *__restrict__ col, FpKeys k,
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_fingerprint(Frontier in, const uint32_t *__restrict__ off, Op op,
                                                         unsigned long long *__restrict__ part) {
     __shared__ uint64_t s[4][kWavesPerBlock];
-    FpOp op{col, k, FpAcc{{1, 1}, {1, 1}}};
     expand_static(in, off, op);
     const uint64_t x[4] = {op.a.f[0], op.a.f[1], op.a.b[0], op.a.b[1]};
     const uint32_t wib = threadIdx.x >> 6;
@@ -153,17 +166,16 @@ __global__ __launch_bounds__(kBlock) void k_fingerprint(Frontier in, const uint3
     }
 }
 
-int check_symmetric(ptg_graph *g, int *sym) {
-    if (g->symmetric >= 0) {
-        *sym = g->symmetric;
-        return PTG_OK;
-    }
+// Runs a fingerprint pass with `op` over the out-CSR; *equal = forward and
+// backward products agree for both key sets.
+template <class Op>
+static int fingerprint_equal(ptg_graph *g, Op op, bool *equal) {
     const uint32_t blocks = g->grid;
     unsigned long long *part = nullptr;
     std::vector<unsigned long long> h(4ull * blocks);
     PTG_HIP(hipMalloc((void **)&part, h.size() * sizeof(unsigned long long)));
     const Frontier fin = g->out.bins.q.with(g->out.bins.d_ctr);
-    k_fingerprint<<<blocks, kBlock, 0, g->stream>>>(fin, g->out.off, g->out.col, fp_random_keys(), part);
+    k_fingerprint<<<blocks, kBlock, 0, g->stream>>>(fin, g->out.off, op, part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, h.size() * sizeof(unsigned long long),
                                             hipMemcpyDeviceToHost, g->stream);
@@ -173,8 +185,29 @@ int check_symmetric(ptg_graph *g, int *sym) {
     uint64_t p[4] = {1, 1, 1, 1};
     for (uint32_t b = 0; b < blocks; b++)
         for (int i = 0; i < 4; i++) p[i] = fp_mul(p[i], h[4ull * b + i]);
-    g->symmetric = (p[0] == p[2] && p[1] == p[3]) ? 1 : 0;
+    *equal = p[0] == p[2] && p[1] == p[3];
+    return PTG_OK;
+}
+
+int check_symmetric(ptg_graph *g, int *sym) {
+    if (g->symmetric < 0) {
+        bool eq = false;
+        int rc = fingerprint_equal(g, FpOp{g->out.col, fp_random_keys(), FpAcc{{1, 1}, {1, 1}}}, &eq);
+        if (rc) return rc;
+        g->symmetric = eq ? 1 : 0;
+    }
     *sym = g->symmetric;
+    return PTG_OK;
+}
+
+int check_weight_symmetric(ptg_graph *g, const uint2 *cw, int *wsym) {
+    if (g->wsymmetric < 0) {
+        bool eq = false;
+        int rc = fingerprint_equal(g, FpWOp{cw, fp_random_keys(), FpAcc{{1, 1}, {1, 1}}}, &eq);
+        if (rc) return rc;
+        g->wsymmetric = eq ? 1 : 0;
+    }
+    *wsym = g->wsymmetric;
     return PTG_OK;
 }
 
@@ -189,6 +222,7 @@ FpKeys fp_random_keys() {
     for (int i = 0; i < 2; i++) {
         k.r[i] = draw();
         k.s[i] = draw();
+        k.t[i] = draw();
     }
     return k;
 }

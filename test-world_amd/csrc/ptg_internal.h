@@ -55,12 +55,16 @@ struct RoundCtr {
     uint32_t lo;                            // SSSP: lower end of the current bucket
     uint32_t fmin_inv;                      // SSSP: ~(smallest value in this frontier); 0 = unknown
     uint32_t ndefer;                        // SSSP: 1 if the pre-pass deferred any vertex
-    uint32_t pad[5];
+    uint32_t ucount;                        // SSSP: 1 if the pre-pass filled uedges_sh
+    uint32_t pulled;                        // SSSP: 1 if the round reading this entry pulled
+    uint32_t pad[3];
     uint32_t found_sh[kShards];             // frontier vertices (any degree)
     uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
     unsigned long long edges_sh[kShards];   // sum of out-degrees of the frontier
+    unsigned long long uedges_sh[kShards];  // SSSP: in-degrees of the vertices that can still improve
+    unsigned long long dedges_sh[kShards];  // SSSP: out-degrees of the deferred frontier vertices
 };
-static_assert(sizeof(RoundCtr) == 512, "RoundCtr layout");
+static_assert(sizeof(RoundCtr) == 768, "RoundCtr layout");
 
 // Counters of one frontier. Bins: S = lane per vertex, M = wave per vertex,
 // L = edge chunks of kChunkEdges spread over workgroups.
@@ -189,7 +193,17 @@ struct ptg_graph {
     // computed yet).
     uint32_t *d_defer = nullptr;
     uint2 *d_cw = nullptr;  // (column, weight) pairs for the default SSSP rounds, built on first use
+    // (source, weight) pairs of every vertex's in-edges, in the offsets of
+    // `out` (a symmetric graph's in-degrees are its out-degrees): the SSSP
+    // pull rounds read them. Built on first use.
+    uint2 *d_cwin = nullptr;
+    // The long lists of the SSSP pull rounds (sssp.hip), built on first use:
+    // their vertices, first 64-edge chunks and the entry of every chunk block.
+    uint32_t *d_long_ids = nullptr, *d_long_base = nullptr, *d_long_block = nullptr;
+    uint32_t n_long = 0, long_chunks = 0, pull_tile_deg = 0;
+    bool long_built = false;
     uint32_t auto_window = 0;
+    uint32_t wmin = 0, wmax = 0;  // smallest and largest edge weight (valid once auto_window is set)
     bool defer_dirty = false;  // the deferred bitmap may hold bits of a failed query
 
     // Batched multi-source state (batch.hip), allocated on the first batch.
@@ -199,12 +213,19 @@ struct ptg_graph {
 
     // Frontier size of every round of the last BFS / SSSP / label-propagation query.
     std::vector<uint64_t> frontiers;
+    uint32_t pulled_rounds = 0;  // rounds of the last BFS / SSSP / label-propagation query that pulled
+    uint64_t pulled_mask = 0;    // which of its rounds 0..63 pulled
+    // SSSP: rounds 0..63 that pulled in the last two queries (all ones until
+    // known) and the queries run, for the pull kernel's launch window.
+    uint64_t pull_hist[2] = {~0ull, ~0ull};
+    uint32_t pull_queries = 0;
     // Rounds of the last BFS [0], SSSP [1] and label-propagation [2] query
     // (0 = none yet): the next query of the kind launches that many rounds + 1
     // before its first readback.
     uint32_t last_rounds[3] = {0, 0, 0};
 
     int symmetric = -1;  // -1 unknown, 0/1 once checked
+    int wsymmetric = -1; // weights symmetric too: -1 unknown, 0/1 once checked
     uint64_t device_bytes = 0;                          // charged against the budget
     uint64_t create_bytes = 0;                          // device_bytes when creation finished
     std::unordered_map<void *, uint64_t> alloc_bytes;   // live allocations and their sizes
@@ -246,6 +267,9 @@ int pr_scratch_setup(int device);
 uint64_t pr_scratch_need(const ptg_graph *g);
 int run_pagerank_noalloc(ptg_graph *g, float damping, int iters, float *out, ptg_stats *st);
 int check_symmetric(ptg_graph *g, int *sym);
+// Whether the weighted multiset of (v, u, w) equals that of (u, v, w), from
+// the graph's (column, weight) pairs cw (once per graph).
+int check_weight_symmetric(ptg_graph *g, const uint2 *cw, int *wsym);
 int ensure_in_edges(ptg_graph *g);
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,
                 uint64_t *edges);
@@ -279,6 +303,8 @@ uint64_t batch_state_need(const ptg_graph *g, bool sssp);
 uint64_t pr_state_need(const ptg_graph *g);
 uint64_t sssp_split_need(const ptg_graph *g);
 void release_sssp_split(ptg_graph *g);
+// Frees the in-edge pairs of the SSSP pull rounds (rebuilt on the next query that pulls).
+void release_sssp_pull(ptg_graph *g);
 void release_pr_state(ptg_graph *g);
 int row_checksums(ptg_graph *g, hipStream_t stream, const uint32_t *rows, uint32_t n, unsigned long long *d_out);
 // Streamed graphs (stream.hip).
@@ -307,6 +333,7 @@ int unpermute(ptg_graph *g, const uint32_t *src, uint32_t *dst);
 struct LaunchTimer {
     ptg_graph *g;
     bool on;
+    uint32_t riders = 0;  // timed launches that belong to the round of the launch before
     explicit LaunchTimer(ptg_graph *gr);
     // Event pair for the next timed launch, or false when timing is off.
     bool next(hipEvent_t *start, hipEvent_t *stop);
@@ -321,6 +348,15 @@ struct LaunchTimer {
             hipExtLaunchKernelGGL(kernel, grid, block, 0, g->stream, a, b, 0, args...);
         else
             hipLaunchKernelGGL(kernel, grid, block, 0, g->stream, args...);
+    }
+    // Same, for a second kernel of the same round: its time adds to the
+    // main kernel's, its launch does not count as another round.
+    template <class K, class... A>
+    void launch_rider(K kernel, dim3 grid, dim3 block, A... args) {
+        const bool timed = on;
+        if (timed) riders++;
+        launch(kernel, grid, block, args...);
+        if (timed && !on) riders--;  // timing switched itself off at its cap
     }
 };
 

@@ -61,6 +61,9 @@ struct RoundEnv {
     uint32_t defer_deg;           // defer vertices of degree > defer_deg (~0u: never)
     uint32_t window;              // ... whose value >= frontier minimum + window
     float bu_beta;                // BFS: bottom-up -> top-down switch factor (0: not BFS)
+    uint32_t pull;                // SSSP: flags pre-passes count the in-edges of the vertices
+    uint32_t pull_wmin;           // ... whose value is above the frontier minimum + pull_wmin,
+    uint32_t pull_gate;           // ... when the previous frontier had at least this many edges
 };
 
 // Saturating a + b.
@@ -72,6 +75,11 @@ __device__ __forceinline__ uint32_t defer_limit(const RoundCtr *in, uint32_t win
     const uint32_t fi = in->fmin_inv;
     return fi ? add_sat(~fi, window) : 0xFFFFFFFFu;
 }
+
+// Values above this bound can still fall in the round that reads `in`:
+// every edge out of the frontier carries at least the frontier minimum plus
+// the smallest weight. Valid when in->fmin_inv != 0.
+__device__ __forceinline__ uint32_t pull_bound(const RoundCtr *in, uint32_t wmin) { return add_sat(~in->fmin_inv, wmin); }
 
 // Per-thread tallies, summed per workgroup at the end of a round.
 struct Acc {
@@ -167,6 +175,7 @@ __device__ __forceinline__ bool bfs_leaves_bottom_up(uint32_t found, uint32_t V,
 
 // Zeroes the ring entry two rounds ahead (done by every round, active or
 // not, so rounds launched past the end of a search always find zeros).
+static_assert(sizeof(RoundCtr) / 4 <= kBlock, "one workgroup clears a ring entry");
 __device__ __forceinline__ void clear_ring_ahead(const RoundEnv &e) {
     if (blockIdx.x == 0 && threadIdx.x < sizeof(RoundCtr) / 4)
         reinterpret_cast<uint32_t *>(e.ring + (e.r + 2) % kRing)[threadIdx.x] = 0u;
@@ -505,6 +514,8 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_
     uint64_t fs = 0, es = 0;
     uint32_t checked = 0;  // ring entries [0, checked) already summed
     g->frontiers.clear();
+    g->pulled_rounds = 0;
+    g->pulled_mask = 0;
     for (;;) {
         for (uint32_t k = 0; k < batch; k++, r++) {
             int rc = launch(r);
@@ -531,6 +542,8 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_
             fs += s.found;
             es += s.edges;
             g->frontiers.push_back(s.found);
+            g->pulled_rounds += c.pulled;
+            if (c.pulled && checked < 64) g->pulled_mask |= 1ull << checked;
             rounds++;
         }
         if (done) break;

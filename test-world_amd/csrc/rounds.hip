@@ -37,6 +37,9 @@ RoundEnv make_env_csr(ptg_graph *g, uint32_t r, const uint32_t *off, const uint3
     e.defer_deg = 0xFFFFFFFFu;
     e.window = 0;
     e.bu_beta = 0.0f;
+    e.pull = 0;
+    e.pull_wmin = 0;
+    e.pull_gate = 0;
     return e;
 }
 
@@ -50,7 +53,19 @@ struct PrepLds {
     uint32_t v[kBuf], n[kBuf];
     uint32_t cnt, wsum[kWavesPerBlock];
     unsigned long long old;
+    unsigned long long usum[kWavesPerBlock], dsum[kWavesPerBlock];
 };
+
+// Whether the pre-pass of round e.r counts the vertices that can still
+// improve for an SSSP pull decision.
+__device__ __forceinline__ bool counts_open(const RoundEnv &e, const RoundCtr *in) {
+    if (!e.pull || in->fmin_inv == 0 || e.r == 0) return false;
+    const RoundCtr *prev = e.ring + (e.r - 1) % kRing;
+    uint64_t pe = 0;
+#pragma unroll
+    for (int i = 0; i < kShards; i++) pe += prev->edges_sh[i];
+    return pe >= e.pull_gate;
+}
 
 // Tiles [t_begin, t_end) of the workgroup, kU per wave per step. Every load
 // of a step (the frontier word or flag byte, offsets v and v + 1) is issued
@@ -68,10 +83,15 @@ __device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, u
     uint32_t *ids = e.cqin + (uint64_t)shard * e.cq_cap;
     uint32_t *bases = e.cbin + (uint64_t)shard * e.cq_cap;
     Acc acc;
+    // SSSP pull rounds: in-edges (= out-degrees, the graph is symmetric) of
+    // the vertices whose value is above the bound
+    const bool count_u = kFlags && counts_open(e, in);
+    const uint32_t bound = count_u ? pull_bound(in, e.pull_wmin) : 0u;
+    uint64_t uacc = 0, dacc = 0;
     if (threadIdx.x == 0) L.cnt = 0;
     __syncthreads();
     for (uint32_t t0 = t_begin; t0 < t_end; t0 += kStep) {
-        uint32_t x[kU], o0[kU], o1[kU];
+        uint32_t x[kU], o0[kU], o1[kU], vv[kU];
 #pragma unroll
         for (uint32_t u = 0; u < kU; u++) {
             const uint32_t p = min(t0 + u * kWavesPerBlock + wib, t_end - 1);
@@ -82,6 +102,7 @@ __device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, u
                 x[u] = e.fin[2 * p + (lane >> 5)];
             o0[u] = e.off[min(v, e.V)];
             o1[u] = e.off[min(v + 1, e.V)];
+            vv[u] = count_u ? e.val[min(v, e.V - 1)] : 0u;  // past V the degree is 0
         }
         uint32_t d[kU];
         bool want[kU];
@@ -99,6 +120,7 @@ __device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, u
                 mem = ok && ((x[u] >> (lane & 31)) & 1u);
             }
             d[u] = mem ? o1[u] - o0[u] : 0u;
+            if (count_u && ok && vv[u] > bound) uacc += o1[u] - o0[u];
             if (kFlags && mem) {
                 acc.found++;
                 acc.edges += d[u];
@@ -110,12 +132,15 @@ __device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, u
             uint32_t val[kU];
 #pragma unroll
             for (uint32_t u = 0; u < kU; u++)
-                val[u] = want[u] && d[u] > e.defer_deg ? e.val[(t0 + u * kWavesPerBlock + wib) * 64 + lane] : 0u;
+                val[u] = want[u] && d[u] > e.defer_deg
+                             ? (count_u ? vv[u] : e.val[(t0 + u * kWavesPerBlock + wib) * 64 + lane])
+                             : 0u;
 #pragma unroll
             for (uint32_t u = 0; u < kU; u++) {
                 const uint32_t p = t0 + u * kWavesPerBlock + wib;
                 const bool defer = want[u] && d[u] > e.defer_deg && val[u] >= lim;
                 want[u] = want[u] && !defer;
+                if (defer) dacc += d[u];
                 const uint64_t md = ballot(defer);
                 if (md) {
                     if ((lane & 31) == 0) {
@@ -178,6 +203,24 @@ __device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, u
         }
     }
     if (kFlags) add_tallies(in, acc);
+    if (count_u) {
+        const uint64_t us = wave_sum_u64(uacc), ds = wave_sum_u64(dacc);
+        if (lane == 0) {
+            L.usum[wib] = us;
+            L.dsum[wib] = ds;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tu = 0, td = 0;
+            for (int w = 0; w < kWavesPerBlock; w++) {
+                tu += L.usum[w];
+                td += L.dsum[w];
+            }
+            if (tu) atomicAdd(&in->uedges_sh[shard], tu);
+            if (td) atomicAdd(&in->dedges_sh[shard], td);
+            if (blockIdx.x == 0) in->ucount = 1u;
+        }
+    }
 }
 
 // Pre-pass of round e.r for flags and bitmap frontiers. Workgroup b handles
@@ -186,8 +229,10 @@ __device__ __forceinline__ void prepare_tiles(const RoundEnv &e, RoundCtr *in, u
 // degree > sd in LDS and appends them to chunk-queue shard b % 16 with one
 // atomic per batch (an LDS scan gives the chunk bases). Members that the
 // round defers (degree > defer_deg, value >= the frontier minimum + window)
-// go to the deferred bitmap instead. A BFS round that stays bottom-up reads
-// the bitmap alone, so its pre-pass has nothing to do.
+// go to the deferred bitmap instead. For SSSP pull decisions a flags
+// pre-pass also sums the in-degrees of the vertices whose value can still
+// fall (above the frontier minimum + the smallest weight). A BFS round that
+// stays bottom-up reads the bitmap alone, so its pre-pass has nothing to do.
 __global__ __launch_bounds__(kBlock) void k_prepare(RoundEnv e) {
     __shared__ PrepLds L;
     RoundCtr *in = e.ring + e.r % kRing;

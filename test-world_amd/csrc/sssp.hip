@@ -162,6 +162,218 @@ __device__ __forceinline__ void sssp_split(const RoundEnv &e, const uint32_t *va
     }
 }
 
+struct PullParams {
+    const uint2 *cwin;   // in-edge (source, weight) pairs
+    uint32_t *val;
+    float pull;          // as RelaxParams::pull
+    uint32_t wmin;       // smallest edge weight
+    uint32_t sent;       // value of unreached vertices
+    const uint32_t *long_ids, *long_base;  // vertices of degree > tile_deg, first chunk of each
+    const uint32_t *long_block;            // entry holding chunk b * kPullBlock, per block b
+    uint32_t n_long, long_chunks;
+    uint32_t tile_deg;                     // lists up to this long are walked in their tile
+};
+
+// Pull rounds: lists of up to tile_deg in-edges are walked by the wave that
+// owns their tile, longer ones from a fixed list in blocks of kPullBlock
+// 64-edge chunks. tile_deg is set per graph so that a tile full of such
+// lists holds at most 4 x E / (waves of the grid) edges, within [sd, 256]:
+// 256 at R-MAT scale 22, 58 at scale 18, where there are fewer tiles than
+// waves and a heavy tile would hold up the round.
+constexpr uint32_t kPullTileMax = 256;
+constexpr uint32_t kPullBlock = 16;
+
+// Pull round (symmetric graph, one unbounded bucket), chosen when the
+// frontier's out-edges outnumber the in-edges of the vertices that can still
+// improve. Only a vertex above `bound` (frontier minimum + smallest weight)
+// can fall, since every edge out of the frontier carries at least that; such
+// an open vertex takes the minimum of val[u] + w over its in-edges (u, w)
+// from reached neighbours. A neighbour outside the frontier already satisfies
+// val[v] <= val[u] + w, so reading it changes nothing, and a neighbour whose
+// value falls during the round joins the next frontier, as an improved vertex
+// does in a push round: the fixed point is the push rounds' one. Improved
+// vertices go to the output flags.
+//
+// Vertices of degree > tile_deg are taken from a fixed list of them (built
+// once per graph), the rest from the tiles of 64 vertices.
+
+// The long lists: the fixed list's 64-edge chunks are cut into blocks of
+// kPullBlock chunks, dealt to the waves round-robin (wave gw takes blocks gw,
+// gw + nw, ...), so the open vertices, wherever they sit in the list, are
+// spread over all waves. A wave skips the chunks of vertices that are not
+// open and lowers every open vertex it touches to the minimum over its part
+// of the list with one atomicMin.
+__device__ __forceinline__ void pull_long(const RoundEnv &e, const PullParams &q, uint32_t bound, const Emit &em,
+                                          Acc &acc, uint32_t gw, uint32_t nw) {
+    const uint32_t lane = lane_id();
+    const uint32_t C = q.long_chunks, n = q.n_long;
+    const uint32_t nblocks = (C + kPullBlock - 1) / kPullBlock;
+    for (uint32_t blk = gw; blk < nblocks; blk += nw) {
+        const uint32_t c1 = min(C, (blk + 1) * kPullBlock);
+        uint32_t c = blk * kPullBlock;
+        uint32_t idx = q.long_block[blk];  // the entry holding chunk c
+        uint32_t k = c - q.long_base[idx];
+        while (c < c1) {
+            // entries [idx, idx + nb), one per lane
+            const uint32_t nb = min(64u, n - idx);
+            uint32_t mv = 0, mb = 0, md = 0, mc = 0;
+            if (lane < nb) {
+                mv = q.long_ids[idx + lane];
+                mb = e.off[mv];
+                md = e.off[mv + 1] - mb;
+                mc = q.val[mv];
+            }
+            for (uint32_t i = 0; i < nb && c < c1; i++) {
+                const uint32_t v = __builtin_amdgcn_readlane(mv, i);
+                const uint32_t vb = __builtin_amdgcn_readlane(mb, i);
+                const uint32_t vd = __builtin_amdgcn_readlane(md, i);
+                const uint32_t vc = __builtin_amdgcn_readlane(mc, i);
+                const uint32_t nch = wave_chunks(vd);
+                if (vc <= bound) {  // not open: skip its chunks in the block
+                    c += min(nch - k, c1 - c);
+                    k = 0;
+                    continue;
+                }
+                uint32_t m = 0xFFFFFFFFu;
+                while (k < nch && c < c1) {
+                    const uint32_t gn = min(min(kGroup, nch - k), c1 - c);
+                    const uint32_t b = k * kWaveChunk;
+                    const uint32_t end = min(vd, b + kWaveChunk * gn);
+                    uint2 x[kGroup];
+#pragma unroll
+                    for (uint32_t j = 0; j < kGroup; j++) {
+                        const uint32_t t = b + 64 * j + lane;
+                        x[j] = t < end ? q.cwin[(uint64_t)vb + t] : make_uint2(0u, 0u);
+                    }
+                    uint32_t cu[kGroup];
+#pragma unroll
+                    for (uint32_t j = 0; j < kGroup; j++) cu[j] = b + 64 * j + lane < end ? q.val[x[j].x] : q.sent;
+#pragma unroll
+                    for (uint32_t j = 0; j < kGroup; j++)
+                        if (cu[j] < q.sent) m = min(m, cu[j] + x[j].y);
+                    k += gn;
+                    c += gn;
+                }
+                k = 0;  // the next entry starts at its first chunk
+                m = wave_min_u32(m);
+                // a candidate below the value read counts as improved without
+                // waiting for the atomic: if another part of the list went
+                // lower, the vertex is in the next frontier anyway
+                const bool improved = lane == 0 && m < vc;
+                if (improved) {
+                    (void)__hip_atomic_fetch_min(&q.val[v], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    e.flout[v] = 1;
+                    acc.fmin_inv = max(acc.fmin_inv, ~m);
+                }
+                mark_any(em, improved);
+            }
+            idx += nb;
+        }
+    }
+}
+
+// A wave's LDS for the edge-parallel walk of its tile's short lists.
+struct PullLds {
+    uint32_t start[64], base[64], best[64];
+};
+
+// The tiles: a wave owns 64 consecutive vertices, and the open ones with
+// 1..tile_deg in-edges are walked edge-parallel over the concatenation of
+// their lists, a lane per edge (consecutive lanes on
+// consecutive edges of a list); the owner is found by binary search over the
+// lists' starts in LDS and its minimum lowered with an LDS atomic. Each value
+// has one writer here and is stored without an atomic.
+__device__ __forceinline__ void pull_tiles(const RoundEnv &e, const uint2 *__restrict__ cwin, uint32_t *val,
+                                           uint32_t bound, uint32_t sent, uint32_t tile_deg, const Emit &em,
+                                           Acc &acc, uint32_t gw, uint32_t nw) {
+    __shared__ PullLds s_pull[kWavesPerBlock];
+    PullLds &L = s_pull[threadIdx.x >> 6];
+    const uint32_t lane = lane_id();
+    for (uint32_t p = gw; p < e.npairs; p += nw) {
+        const uint32_t v = p * 64 + lane;
+        const uint32_t cur = v < e.V ? val[v] : 0u;
+        const bool open = v < e.V && cur > bound;
+        if (ballot(open) == 0) continue;
+        uint32_t b = 0, d = 0;
+        if (open) {
+            b = e.off[v];
+            d = e.off[v + 1] - b;
+        }
+        const bool sm = open && d > 0 && d <= tile_deg;
+        const uint64_t msm = ballot(sm);
+        if (msm == 0) continue;
+        const uint32_t n = (uint32_t)__popcll(msm);
+        const uint32_t k = rank_below(msm);
+        const uint32_t ds = sm ? d : 0u;
+        const uint32_t incl = wave_incl_scan_u32(ds);
+        const uint32_t T = read_lane(incl, 63);
+        if (sm) {
+            L.start[k] = incl - ds;
+            L.base[k] = b;
+            L.best[k] = 0xFFFFFFFFu;
+        }
+        wave_lds_sync();
+        for (uint32_t t0 = 0; t0 < T; t0 += kGroup * 64) {
+            uint32_t own[kGroup];
+            uint2 x[kGroup];
+#pragma unroll
+            for (uint32_t j = 0; j < kGroup; j++) {
+                const uint32_t t = t0 + 64 * j + lane;
+                uint32_t lo = 0, hi = n;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (L.start[mid] <= t) lo = mid; else hi = mid;
+                }
+                own[j] = lo;
+                x[j] = t < T ? cwin[(uint64_t)L.base[lo] + (t - L.start[lo])] : make_uint2(0u, 0u);
+            }
+            uint32_t cu[kGroup];
+#pragma unroll
+            for (uint32_t j = 0; j < kGroup; j++) cu[j] = t0 + 64 * j + lane < T ? val[x[j].x] : sent;
+#pragma unroll
+            for (uint32_t j = 0; j < kGroup; j++)
+                if (cu[j] < sent) atomicMin(&L.best[own[j]], cu[j] + x[j].y);
+        }
+        wave_lds_sync();
+        const uint32_t best = sm ? L.best[k] : 0xFFFFFFFFu;
+        wave_lds_sync();  // the next tile rewrites the arrays
+        const bool improved = sm && best < cur;
+        if (improved) {
+            val[v] = best;
+            e.flout[v] = 1;
+            acc.fmin_inv = max(acc.fmin_inv, ~best);
+        }
+        mark_any(em, improved);
+    }
+}
+
+// The pre-pass of a round counts open vertices (and queues their long lists)
+// only when the previous frontier had at least E / kPullGateShare edges: a
+// round after a small frontier pushes few edges and would not pull.
+constexpr uint64_t kPullGateShare = 64;
+
+// Whether a round pulls: the pre-pass counted the in-edges of the vertices
+// that can still improve (uedges), and they times `factor` are fewer than
+// the out-edges a push round would scan (the frontier's, less those of its
+// deferred vertices).
+__device__ __forceinline__ bool pull_pays(const RoundCtr *in, uint64_t frontier_edges, float factor) {
+    if (!in->ucount) return false;
+    uint64_t u = 0, d = 0;
+#pragma unroll
+    for (int i = 0; i < kShards; i++) {
+        u += in->uedges_sh[i];
+        d += in->dedges_sh[i];
+    }
+    return (double)u * factor < (double)(frontier_edges - min(d, frontier_edges));
+}
+
+// Whether round e.r pulls (the push and pull kernels of the round decide
+// alike): a frontier held as flags (the pre-pass counted the open in-edges)
+// with a pull kernel behind the round (factor > 0) that pays.
+__device__ __forceinline__ bool round_pulls(const RoundCtr *in, const InSummary &s, uint32_t qcap, float factor) {
+    return factor > 0 && s.found > 0 && s.edges > qcap && in->mode == kModeFlags && pull_pays(in, s.edges, factor);
+}
+
 struct RelaxParams {
     uint32_t *val;
     const uint32_t *wt;  // weights of the CSR the rounds expand (light edges)
@@ -173,6 +385,8 @@ struct RelaxParams {
     Frontier hbins;      // static degree bins of the heavy CSR
     uint32_t defer_deg;  // ~0u: no deferral
     uint32_t window;
+    float pull;          // > 0: a pull kernel follows each round, which pulls when
+                         // open in-edges * pull < the push round's out-edges
 };
 
 // Heavy-edge round: every vertex whose final value lies in the bucket
@@ -243,7 +457,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxPara
         const uint32_t lim = p.defer_deg != 0xFFFFFFFFu ? defer_limit(in, p.window) : 0xFFFFFFFFu;
         Relax<kWeighted, kWeighted && !kBuckets> op{e.off,   e.col,   p.wt, p.cw,  p.val,       e.fout,
                                                      e.flout, p.far,   thr,  p.defer_deg, lim};
-        expand_frontier(e, in, op, em, acc);
+        // a pulling round is expanded by k_pull_round, launched behind this one
+        if (!(kWeighted && !kBuckets && round_pulls(in, s, e.qcap, p.pull))) expand_frontier(e, in, op, em, acc);
         if (lim != 0xFFFFFFFFu) requeue_deferred(e, in, op, em, acc);
     } else if constexpr (kBuckets) {
         if (heavy)
@@ -256,27 +471,70 @@ __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxPara
     add_tallies(out, acc);
 }
 
+// The pull expansion of round e.r, launched behind k_relax_round, which has
+// done the rest of the round (output header, deferred vertices, clears);
+// returns at once when the round does not pull. A kernel of its own leaves
+// the push kernel's register allocation as it was.
+__global__ __launch_bounds__(kBlock, 8) void k_pull_round(RoundEnv e, PullParams q) {
+    RoundCtr *in = e.ring + e.r % kRing;
+    const InSummary s = summarize(in);
+    if (!round_pulls(in, s, e.qcap, q.pull)) return;
+    RoundCtr *out = e.ring + (e.r + 1) % kRing;
+    if (blockIdx.x == 0 && threadIdx.x == 0) in->pulled = 1u;
+    const Emit em = make_emit(e, out, kModeFlags);
+    Acc acc;
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const uint32_t bound = pull_bound(in, q.wmin);
+    pull_long(e, q, bound, em, acc, gw, nw);
+    pull_tiles(e, q.cwin, q.val, bound, q.sent, q.tile_deg, em, acc, gw, nw);
+    add_tallies(out, acc);
+}
+
+// Rounds that launch the pull kernel: those within one of a round that pulled
+// in either of the last two queries (rounds from 64 on always), and every
+// round of each 16th query, so that a change of source or graph shape is
+// seen again. A round without the kernel pushes, with the same result.
+static uint64_t pull_window(ptg_graph *g) {
+    if (g->pull_queries++ % 16 == 0) return ~0ull;
+    const uint64_t h = g->pull_hist[0] | g->pull_hist[1];
+    return h | (h << 1) | (h >> 1);
+}
+
 template <bool kWeighted, class Finish>
-static int run_relax_rounds(ptg_graph *g, RelaxParams p, const uint32_t *off, const uint32_t *col, LaunchTimer &timer,
-                            uint32_t *iters, uint64_t *fsum, Finish finish, bool *finished) {
-    return drive_rounds(
+static int run_relax_rounds(ptg_graph *g, RelaxParams p, const PullParams &q, const uint32_t *off,
+                            const uint32_t *col, LaunchTimer &timer, uint32_t *iters, uint64_t *fsum, Finish finish,
+                            bool *finished) {
+    const uint64_t window = p.pull > 0 ? pull_window(g) : 0ull;
+    const float factor = p.pull;
+    int rc = drive_rounds(
         g,
         [&](uint32_t r) -> int {
             RoundEnv env = make_env_csr(g, r, off, col);
             env.val = p.val;
             env.defer_deg = p.defer_deg;
             env.window = p.window;
+            p.pull = factor > 0 && (r >= 64 || ((window >> r) & 1)) ? factor : 0.0f;
+            env.pull = p.pull > 0 ? 1u : 0u;
+            env.pull_wmin = q.wmin;
+            env.pull_gate = (uint32_t)std::min<uint64_t>(g->E / kPullGateShare, 0xFFFFFFFFu);
             int rc = launch_prepare(g, env);
             if (rc) return rc;
             if (p.split)
                 timer.launch(k_relax_round<kWeighted, true>, dim3(g->grid), dim3(kBlock), env, p);
             else
                 timer.launch(k_relax_round<kWeighted, false>, dim3(g->grid), dim3(kBlock), env, p);
+            if (p.pull > 0) timer.launch_rider(k_pull_round, dim3(g->grid), dim3(kBlock), env, q);
             PTG_HIP(hipGetLastError());
             return PTG_OK;
         },
         [](const RoundCtr &c) { return c.any == 0 && c.far_any == 0 && c.need_heavy == 0; },
         g->last_rounds[kWeighted ? 1 : 2], iters, fsum, nullptr, finish, finished);
+    if (!rc && factor > 0) {
+        g->pull_hist[1] = g->pull_hist[0];
+        g->pull_hist[0] = g->pulled_mask;
+    }
+    return rc;
 }
 
 // Counts the light edges (weight < delta) of every vertex: one wave each.
@@ -385,8 +643,21 @@ static int ensure_split(ptg_graph *g, uint32_t delta) {
     return PTG_OK;
 }
 
+// Pull rounds run on a symmetric graph with one unbounded bucket, unless
+// switched off, and only where no path value plus an edge can reach the
+// unreached value (every value is a simple path's length, at most
+// 1 + (V - 1) * wmax), so a candidate never wraps.
+static bool pull_wanted(const ptg_graph *g, bool split) {
+    return !split && g->cfg.sssp_pull >= 0 && g->symmetric != 0;
+}
+
+static bool pull_in_range(const ptg_graph *g) {
+    return 1 + (uint64_t)g->V * g->wmax + g->wmax < (uint64_t)g->cfg.sssp_sentinel;
+}
+
 void release_sssp_split(ptg_graph *g) {
     free_split(g);
+    release_sssp_pull(g);
     gfree(g, g->d_defer);
     g->d_defer = nullptr;
     gfree(g, g->d_cw);
@@ -398,18 +669,34 @@ uint64_t sssp_split_need(const ptg_graph *g) {
     uint64_t need = g->d_defer ? 0 : 4ull * g->nwords;
     const uint32_t delta = g->cfg.sssp_delta ? g->cfg.sssp_delta : kUnbounded;
     if (delta == kUnbounded && !g->d_cw) need += 8ull * std::max<uint64_t>(g->E, 1);
+    if (pull_wanted(g, delta != kUnbounded) && g->wsymmetric != 1 && !g->d_cwin)
+        need += 8ull * std::max<uint64_t>(g->E, 1) + 4ull * g->V + 4;
+    if (pull_wanted(g, delta != kUnbounded) && !g->long_built) need += 12ull * g->V + 12;
     if (delta != kUnbounded && !(g->split_delta == delta && g->light.off))
         need += 8ull * g->E + 8ull * (g->V + 1) + 4ull * g->V + (64u << 10);
     return need;
 }
 
-// Sum of the weights, over 16 accumulators.
+// Sum of the weights over 16 accumulators, smallest weight in acc[16],
+// largest in acc[17].
 __global__ __launch_bounds__(kBlock) void k_weight_sum(const uint32_t *__restrict__ w, uint64_t E,
                                                        unsigned long long *acc) {
     uint64_t sum = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < E; i += (uint64_t)gridDim.x * kBlock) sum += w[i];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < E; i += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t x = w[i];
+        sum += x;
+        lo = min(lo, x);
+        hi = max(hi, x);
+    }
     sum = wave_sum_u64(sum);
-    if (lane_id() == 0) atomicAdd(&acc[(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kShards], sum);
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    if (lane_id() == 0) {
+        atomicAdd(&acc[(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kShards], sum);
+        atomicMin(&acc[16], (unsigned long long)lo);
+        atomicMax(&acc[17], (unsigned long long)hi);
+    }
 }
 
 // (column, weight) of every edge side by side, for the default rounds.
@@ -445,8 +732,11 @@ static int ensure_defer(ptg_graph *g) {
         g->defer_dirty = false;
     }
     if (g->auto_window) return PTG_OK;
-    unsigned long long sums[kShards];
-    if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(g->d_acc), 2 * kShards, 0u))) return rc;
+    unsigned long long sums[kShards + 2];
+    uint32_t *acc = reinterpret_cast<uint32_t *>(g->d_acc);
+    const FillSpan z[3] = {FillSpan{acc, 2 * kShards, 0u}, FillSpan{acc + 2 * kShards, 2, 0xFFFFFFFFu},
+                           FillSpan{acc + 2 * kShards + 2, 2, 0u}};
+    if ((rc = fill_spans(g, z, 3))) return rc;
     if (g->E) {
         k_weight_sum<<<g->grid, kBlock, 0, g->stream>>>(g->d_w, g->E, g->d_acc);
         PTG_HIP(hipGetLastError());
@@ -455,8 +745,115 @@ static int ensure_defer(ptg_graph *g) {
     uint64_t total = 0;
     for (int i = 0; i < kShards; i++) total += sums[i];
     const uint64_t mean = g->E ? total / g->E : 0;
+    g->wmin = g->E ? (uint32_t)sums[kShards] : 0u;
+    g->wmax = g->E ? (uint32_t)sums[kShards + 1] : 0u;
     g->auto_window = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((mean + 8) / 16, 1u << 30));
     return PTG_OK;
+}
+
+// In-edge (source, weight) pairs of a symmetric graph, in the out-CSR's
+// offsets: one wave per source vertex u, every out-edge (u -> v, w) takes the
+// next slot of v's list. A slot past the list means in- and out-degrees
+// differ (the graph is not symmetric after all): *bad is set, nothing is
+// written out of the list.
+__global__ __launch_bounds__(kBlock) void k_in_pairs(uint32_t V, const uint32_t *__restrict__ off,
+                                                     const uint2 *__restrict__ cw, uint32_t *__restrict__ cursor,
+                                                     uint2 *__restrict__ cwin, uint32_t *__restrict__ bad) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    for (uint32_t u = gw; u < V; u += nw) {
+        const uint32_t end = off[u + 1];
+        for (uint32_t i = off[u] + lane; i < end; i += 64) {
+            const uint2 x = cw[i];
+            const uint32_t slot = atomicAdd(&cursor[x.x], 1u);
+            const uint32_t lo = off[x.x], hi = off[x.x + 1];
+            if (slot < hi - lo)
+                cwin[(uint64_t)lo + slot] = make_uint2(u, x.y);
+            else
+                *bad = 1u;
+        }
+    }
+}
+
+void release_sssp_pull(ptg_graph *g) {
+    gfree(g, g->d_cwin);
+    gfree(g, g->d_long_ids);
+    gfree(g, g->d_long_base);
+    gfree(g, g->d_long_block);
+    g->d_cwin = nullptr;
+    g->d_long_ids = g->d_long_base = g->d_long_block = nullptr;
+    g->n_long = g->long_chunks = 0;
+    g->long_built = false;
+}
+
+// Vertices of degree > tile_deg with the index of their first 64-edge
+// chunk, in vertex order, and for every block of kPullBlock chunks the entry
+// holding its first chunk (once per graph).
+static int ensure_long_list(ptg_graph *g) {
+    if (g->long_built) return PTG_OK;
+    const uint32_t V = g->V;
+    const uint64_t waves = (uint64_t)g->grid * kWavesPerBlock;
+    const uint32_t tile_deg = (uint32_t)std::min<uint64_t>(
+        kPullTileMax, std::max<uint64_t>(g->cfg.small_degree, 4 * g->E / (64 * waves)));
+    std::vector<uint32_t> off((size_t)V + 1), ids, base, block;
+    int rc = copy_sync(g, off.data(), g->out.off, 4ull * (V + 1), hipMemcpyDeviceToHost);
+    if (rc) return rc;
+    uint64_t chunks = 0;
+    for (uint32_t v = 0; v < V; v++) {
+        const uint32_t d = off[v + 1] - off[v];
+        if (d <= tile_deg) continue;
+        const uint64_t nch = (d + kWaveChunk - 1) / kWaveChunk;
+        for (uint64_t c = (chunks + kPullBlock - 1) / kPullBlock * kPullBlock; c < chunks + nch; c += kPullBlock)
+            block.push_back((uint32_t)ids.size());
+        ids.push_back(v);
+        base.push_back((uint32_t)chunks);
+        chunks += nch;
+    }
+    const uint64_t n = ids.size(), nb = block.size();
+    auto up = [&](uint32_t **dst, const std::vector<uint32_t> &src) -> int {
+        int r = galloc(g, (void **)dst, 4 * std::max<uint64_t>(src.size(), 1));
+        if (!r && !src.empty()) r = copy_sync(g, *dst, src.data(), 4 * src.size(), hipMemcpyHostToDevice);
+        return r;
+    };
+    if ((rc = up(&g->d_long_ids, ids)) || (rc = up(&g->d_long_base, base)) || (rc = up(&g->d_long_block, block))) {
+        gfree(g, g->d_long_ids);
+        gfree(g, g->d_long_base);
+        gfree(g, g->d_long_block);
+        g->d_long_ids = g->d_long_base = g->d_long_block = nullptr;
+        return rc;
+    }
+    (void)nb;
+    g->n_long = (uint32_t)n;
+    g->pull_tile_deg = tile_deg;
+    g->long_chunks = (uint32_t)chunks;  // < E / 64 + V
+    g->long_built = true;
+    return PTG_OK;
+}
+
+// Builds the in-edge pairs once (needs the packed out-pairs). On a graph that
+// turns out not to be symmetric the pairs are dropped and the graph is marked
+// so.
+static int ensure_pull(ptg_graph *g) {
+    if (g->d_cwin) return PTG_OK;
+    int rc;
+    uint32_t *cursor = nullptr;
+    if ((rc = galloc(g, (void **)&g->d_cwin, 8ull * std::max<uint64_t>(g->E, 1)))) return rc;
+    if ((rc = galloc(g, (void **)&cursor, 4ull * g->V + 4))) {
+        release_sssp_pull(g);
+        return rc;
+    }
+    uint32_t bad = 0;
+    if (!(rc = fill_u32(g, cursor, (uint64_t)g->V + 1, 0u))) {
+        if (g->E) k_in_pairs<<<g->grid, kBlock, 0, g->stream>>>(g->V, g->out.off, g->d_cw, cursor, g->d_cwin,
+                                                                 cursor + g->V);
+        if (hipGetLastError() != hipSuccess) rc = fail(PTG_ERR_HIP, "k_in_pairs launch failed");
+        if (!rc) rc = copy_sync(g, &bad, cursor + g->V, 4, hipMemcpyDeviceToHost);
+    }
+    gfree(g, cursor);
+    if (rc || bad) release_sssp_pull(g);
+    if (!rc && bad) g->symmetric = 0;
+    return rc;
 }
 
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
@@ -466,6 +863,19 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     const bool split = delta != kUnbounded;
     if (split ? (rc = ensure_split(g, delta)) : (rc = ensure_packed(g))) return rc;
     if ((rc = ensure_defer(g))) return rc;
+    // in-edge pairs for pull rounds: the out-pairs themselves when every
+    // edge's reverse carries the same weight, else the built in-pairs
+    const uint2 *cwin = nullptr;
+    if (pull_wanted(g, split)) {
+        int sym = 0, wsym = 0;
+        if ((rc = check_symmetric(g, &sym))) return rc;
+        if (sym && pull_in_range(g)) {
+            if ((rc = check_weight_symmetric(g, g->d_cw, &wsym))) return rc;
+            if (!wsym && (rc = ensure_pull(g))) return rc;
+            if ((rc = ensure_long_list(g))) return rc;
+            cwin = wsym ? g->d_cw : g->d_cwin;
+        }
+    }
     const uint32_t defer_deg = g->cfg.sssp_defer_degree ? g->cfg.sssp_defer_degree : kDefaultDeferDegree;
     const uint32_t window = g->cfg.sssp_defer_window ? g->cfg.sssp_defer_window : g->auto_window;
     const uint32_t *off = split ? g->light.off : g->out.off;
@@ -500,8 +910,12 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     uint64_t fsum = 0;
     RelaxParams p{dist, split ? g->d_wl : g->d_w, split ? nullptr : g->d_cw, g->d_far, delta, split ? 1u : 0u,
                   split ? g->heavy.off : nullptr, split ? g->heavy.col : nullptr, split ? g->d_wh : nullptr,
-                  split ? g->heavy.bins.q.with(g->heavy.bins.d_ctr) : Frontier{}, defer_deg, window};
-    if ((rc = run_relax_rounds<true>(g, p, off, col, timer, &iters, &fsum, finish, &finished))) {
+                  split ? g->heavy.bins.q.with(g->heavy.bins.d_ctr) : Frontier{}, defer_deg, window,
+                  cwin ? (g->cfg.sssp_pull > 0 ? g->cfg.sssp_pull : 1.0f) : 0.0f};
+    const PullParams q{cwin,          dist,           p.pull,          g->wmin,  g->cfg.sssp_sentinel,
+                       g->d_long_ids, g->d_long_base, g->d_long_block, g->n_long, g->long_chunks,
+                       g->pull_tile_deg};
+    if ((rc = run_relax_rounds<true>(g, p, q, off, col, timer, &iters, &fsum, finish, &finished))) {
         g->defer_dirty = true;
         return rc;
     }
@@ -514,6 +928,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
         st->kernel_ns = (uint64_t)((double)ms * 1e6);
         st->iterations = iters;
         st->frontier_sum = fsum;
+        st->pull_rounds = g->pulled_rounds;
         if ((rc = timer.collect(st))) return rc;
         reach_stats_read(g, &st->reached, &st->edges_traversed);
     }
@@ -551,8 +966,8 @@ int run_label_propagation(ptg_graph *g, uint32_t *label, LaunchTimer &timer, uin
     PTG_HIP(hipGetLastError());
     RelaxParams p{label, nullptr, nullptr, g->d_far, 0u, 0u, nullptr, nullptr, nullptr, Frontier{}, 0xFFFFFFFFu, 0u};
     bool finished = false;
-    return run_relax_rounds<false>(g, p, g->out.off, g->out.col, timer, iters, fsum, [](bool) { return PTG_OK; },
-                                   &finished);
+    return run_relax_rounds<false>(g, p, PullParams{}, g->out.off, g->out.col, timer, iters, fsum,
+                                   [](bool) { return PTG_OK; }, &finished);
 }
 
 }  // namespace ptg

@@ -78,7 +78,7 @@ class Config(ctypes.Structure):
         ("sssp_defer_window", ctypes.c_uint32),
         ("stream_pinned", ctypes.c_uint32),
         ("batch_sssp_grid", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("sssp_pull", ctypes.c_float),
     ]
 
 
@@ -95,7 +95,7 @@ class Stats(ctypes.Structure):
         ("main_launches", ctypes.c_uint32),
         ("chunk_loads", ctypes.c_uint32),
         ("chunk_hits", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("pull_rounds", ctypes.c_uint32),
     ]
 
     def as_dict(self):

@@ -36,6 +36,9 @@ CONFIGS = {
     "defer_all": {"small_degree": 1, "large_degree": 64, "sssp_defer_degree": 1, "sssp_defer_window": 1},
     "defer_off": {"sssp_defer_degree": 0xFFFFFFFF},
     "defer_split": {"sssp_delta": 24, "sssp_defer_degree": 8, "sssp_defer_window": 1},
+    # SSSP pull rounds in every large round / never
+    "pull_always": {"sssp_pull": 1e-9},
+    "pull_never": {"sssp_pull": -1.0},
 }
 
 

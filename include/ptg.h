@@ -110,6 +110,13 @@ typedef struct ptg_config {
                                      < the frontier's out-edges; the distances are the same.
                                      0 = auto (1.0), < 0 = never (the in-edge pairs, 8 bytes per
                                      edge, are then not built) */
+    float batch_pull;             /* batched SSSP on a symmetric graph: round r of a batch pulls
+                                     (every vertex takes, for every source, the minimum over its
+                                     in-edges, one atomic per edge chunk instead of one per
+                                     improving edge) when round r of the graph's previous batch
+                                     expanded >= E x batch_pull edges and round r - 1 of this one
+                                     >= E x batch_pull / 8; the distances are the same.
+                                     0 = auto (0.25), < 0 = never */
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
@@ -201,7 +208,7 @@ PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
 /* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta,
  * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid, batch_sssp_grid,
- * batch_td_to_bu, batch_chunk, stream_pinned, PTG_FLAG_TIMING and, for a
+ * batch_td_to_bu, batch_chunk, batch_pull, stream_pinned, PTG_FLAG_TIMING and, for a
  * streamed graph, chunk_bytes (the chunks are cut again and the slot cache
  * resized; PTG_ERR_BUDGET, with the previous size kept, when fewer than two
  * slots would fit). Every other field must equal the graph's

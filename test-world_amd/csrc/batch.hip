@@ -23,6 +23,17 @@
 // ptGraph/gpu_kernels.cu:152-172), so every source reaches the same fixed
 // point as ptg_sssp.
 //
+// Pull rounds (batch_pull). On a symmetric graph, a round in which most
+// vertices were improved for some source can instead pull: every vertex w
+// takes, for every source, the minimum of its distance and dist[u] + w(u, w)
+// over its in-edges (u, w) from reached u, chunk by chunk, and applies it
+// with one atomicMin per chunk instead of one per improving edge. A pull
+// relaxes every edge a push round would (and edges out of unchanged vertices,
+// whose candidates were applied before), so push and pull rounds in any mix
+// reach the same fixed point. Which rounds pull is learned per graph: round r
+// pulls when round r of the previous batch expanded at least E x batch_pull
+// edges and round r - 1 of this one at least an eighth of that.
+//
 // Rounds run as hipGraph replays of a fixed number of levels/rounds; every
 // kernel reads its round number from device memory and returns at once when
 // the search has ended, so a replay past the end is harmless.
@@ -41,6 +52,7 @@ constexpr uint32_t kBfsReplay = 8;     // BFS levels per graph replay
 constexpr uint32_t kSsspReplay = 16;   // SSSP rounds per graph replay
 constexpr uint32_t kLevelCap = 32;     // BFS levels kept as masks; deeper -> per-source path
 constexpr uint32_t kBatchChunk = 128;  // SSSP: default edges per static chunk
+constexpr float kBatchPull = 0.25f;    // SSSP: default batch_pull
 
 // Counters of one level / round, 16-way sharded.
 struct BCtr {
@@ -76,6 +88,10 @@ struct BatchEnv {
     const uint32_t *ch_v;      // SSSP: static chunk list, vertex of each chunk
     const uint32_t *ch_b;      //   and its first edge
     uint32_t nch, chunk;       // chunks; edges per chunk
+    const uint2 *cwin;         // SSSP pull rounds: in-edge (source, weight) pairs; null = push only
+    const uint8_t *pmode;      // per round (< kBRing): pull when the previous batch's round was dense
+    unsigned long long pull_prev;  // and the previous round of this batch expanded >= this many edges
+    uint32_t sentinel;         // SSSP: unreached value
     BCtr *ring;
     BParams *p;
     uint32_t sd;
@@ -90,6 +106,9 @@ struct BatchState {
     uint32_t *dist = nullptr;
     uint32_t *ch_v = nullptr, *ch_b = nullptr;
     uint32_t nch = 0, chunk = 0;
+    uint8_t *pmode = nullptr;                   // SSSP pull rounds learned from the previous batch
+    uint8_t h_pmode[kBRing] = {};
+    BCtr *h_ring = nullptr;                     // pinned copy of the counter ring (pull learning)
     uint32_t *out = nullptr;   // n x V staging for host outputs
     uint32_t *srcs = nullptr;  // device copy of the sources
     uint32_t *h_srcs = nullptr;
@@ -471,6 +490,140 @@ __device__ __forceinline__ void relax_chunk(const BatchEnv &e, const ChunkHead &
     }
 }
 
+// Whether round R pulls (the push kernel then leaves the expansion to
+// k_mssp_pull): an expand round of a graph with in-pairs, learned dense in
+// the previous batch, whose previous round expanded enough edges (munf of
+// the input entry).
+__device__ __forceinline__ bool pull_round(const BatchEnv &e, uint32_t R, const BSum &s) {
+    return e.cwin && s.nf > 0 && R < kBRing && e.pmode[R] && s.munf >= e.pull_prev;
+}
+
+// The loads a pull chunk needs before its rows: the vertex's distance in
+// this lane and the first 64 in-edge pairs (indexes clamped into the chunk).
+struct PullHead {
+    uint32_t w, eb, ee;
+    uint32_t dw;
+    uint32_t mu, mwt;
+};
+
+__device__ __forceinline__ PullHead pull_head(const BatchEnv &e, uint32_t w, uint32_t eb, uint32_t ee) {
+    const uint32_t lane = lane_id();
+    PullHead h;
+    h.w = uniform(w);
+    h.eb = uniform(eb);
+    h.ee = uniform(ee);
+    h.dw = e.dist[(uint64_t)h.w * 64 + lane];
+    const uint2 pr = e.cwin[min(h.eb + lane, h.ee - 1)];
+    h.mu = pr.x;
+    h.mwt = pr.y;
+    return h;
+}
+
+// Pulls one chunk [eb, ee) of w's in-edges: lane k takes
+// m = min(dist[w][k], dist[u][k] + weight) over the chunk's reached u and, if
+// m is lower, applies it with one atomicMin and sets w's frontier bit (far
+// bit at or above thr). Rows are read in groups of kP edges with the next
+// group in flight, as in relax_chunk. Pulls run only where no path plus an
+// edge can reach the sentinel, so no candidate wraps.
+__device__ __forceinline__ void pull_chunk(const BatchEnv &e, const PullHead &h, uint32_t thr,
+                                           unsigned long long *nxt, uint64_t &t_nf, uint64_t &t_far) {
+    constexpr uint32_t kP = 4;
+    const uint32_t lane = lane_id();
+    uint32_t m = h.dw;
+    uint32_t mu = h.mu, mwt = h.mwt;
+    for (uint32_t base = h.eb; base < h.ee; base += 64) {
+        const uint32_t cnt = min(64u, h.ee - base);
+        if (base != h.eb) {
+            const uint2 pr = e.cwin[min(base + lane, h.ee - 1)];
+            mu = pr.x;
+            mwt = pr.y;
+        }
+        auto load = [&](uint32_t q, uint32_t (&r)[kP]) {
+#pragma unroll
+            for (uint32_t t = 0; t < kP; t++)
+                r[t] = e.dist[(uint64_t)read_lane(mu, min(q + t, cnt - 1)) * 64 + lane];
+        };
+        auto take = [&](uint32_t q, const uint32_t (&r)[kP]) {
+#pragma unroll
+            for (uint32_t t = 0; t < kP; t++) {
+                const uint32_t c = r[t] + read_lane(mwt, min(q + t, cnt - 1));
+                if (q + t < cnt && r[t] < e.sentinel && c < m) m = c;
+            }
+        };
+        uint32_t ra[kP], rb[kP];
+        load(0, ra);
+        for (uint32_t q = 0;; q += 2 * kP) {
+            if (q + kP >= cnt) {
+                take(q, ra);
+                break;
+            }
+            load(q + kP, rb);
+            take(q, ra);
+            if (q + 2 * kP >= cnt) {
+                take(q + kP, rb);
+                break;
+            }
+            load(q + 2 * kP, ra);
+            take(q + kP, rb);
+        }
+    }
+    const bool imp = m < h.dw;
+    if (imp) atomicMin(&e.dist[(uint64_t)h.w * 64 + lane], m);
+    const unsigned long long mn = ballot(imp && m < thr);
+    const unsigned long long mf = ballot(imp && m >= thr);
+    if (lane == 0) {
+        if (mn) atomicOr(&nxt[h.w], mn);
+        if (mf) atomicOr(&e.far[h.w], mf);
+    }
+    t_nf |= mn != 0;
+    t_far |= mf != 0;
+}
+
+// One pull round (after k_mssp_round of the same round, which did the
+// round's resets): every chunk of the static chunk list is pulled, the next
+// chunk's head loads issued while the current one's rows are read. The
+// round's edge count (munf of the output entry) is that of the chunks whose
+// vertex is in the frontier, the edges a push round would have expanded.
+__global__ __launch_bounds__(kBlock, 8) void k_mssp_pull(BatchEnv e, uint32_t j) {
+    const uint32_t R = e.p->base + j;
+    const BSum s = bsum(e.ring + R % kBRing);
+    if (!pull_round(e, R, s)) return;
+    BCtr *out = e.ring + (R + 1) % kBRing;
+    const uint32_t thr = e.ring[R % kBRing].thr;
+    const unsigned long long *cur = rot(e, R);
+    unsigned long long *nxt = rot(e, R + 1);
+    const uint32_t lane = lane_id();
+    const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    uint64_t t_nf = 0, t_far = s.far ? 1 : 0, t_vis = 0;
+    for (uint64_t i0 = 0; gw + nw * i0 < e.nch; i0 += 64) {
+        const uint64_t c = gw + nw * (i0 + lane);
+        uint32_t v = 0, cb = 0, ce = 0;
+        if (c < e.nch) {
+            v = e.ch_v[c];
+            cb = e.ch_b[c];
+            ce = (uint32_t)min((uint64_t)e.off[v + 1], (uint64_t)cb + e.chunk);
+            if (cur[v]) t_vis += ce - cb;
+        }
+        uint64_t mm = ballot(c < e.nch);
+        auto head = [&]() {
+            const uint32_t q = lowest_lane(mm);
+            mm &= mm - 1;
+            return pull_head(e, read_lane(v, q), read_lane(cb, q), read_lane(ce, q));
+        };
+        PullHead ha = head();
+        for (;;) {
+            const bool more = mm != 0;
+            PullHead hb = ha;
+            if (more) hb = head();
+            pull_chunk(e, ha, thr, nxt, t_nf, t_far);
+            if (!more) break;
+            ha = hb;
+        }
+    }
+    bctr_add(out, t_nf, 0, t_vis, t_far);
+}
+
 // One SSSP round: expand the frontier masks (near), or, when none is left,
 // raise the threshold by delta and move far bits below it into the frontier.
 __global__ __launch_bounds__(kBlock, 8) void k_mssp_round(BatchEnv e, uint32_t j) {
@@ -497,7 +650,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_mssp_round(BatchEnv e, uint32_t j
     const uint64_t nt = (uint64_t)gridDim.x * kBlock;
     for (uint64_t v = gt; v < e.nva; v += nt) clr[v] = 0;
     uint64_t t_nf = 0, t_mf = 0, t_far = 0, t_vis = 0;
-    if (expand) {
+    if (expand && pull_round(e, R, s)) {
+        // k_mssp_pull expands this round
+    } else if (expand) {
         if (s.far) t_far = 1;  // the far set is untouched by an expand round
         // static chunks of at most e.chunk edges, dealt round-robin to the
         // waves (chunk gw + nw * i); lanes test 64 chunks' masks at once and
@@ -654,6 +809,13 @@ static int ensure_batch(ptg_graph *g, bool sssp) {
     if (!sssp && !b->lm && (rc = galloc(g, (void **)&b->lm, 8 * V * kLevelCap))) return rc;
     if (sssp) {
         if (!b->far && (rc = galloc(g, (void **)&b->far, 8 * V))) return rc;
+        if (!b->pmode) {
+            if ((rc = galloc(g, (void **)&b->pmode, kBRing))) return rc;
+            memset(b->h_pmode, 0, sizeof b->h_pmode);
+            if ((rc = copy_sync(g, b->pmode, b->h_pmode, kBRing, hipMemcpyHostToDevice))) return rc;
+        }
+        if (!b->h_ring && hipHostMalloc((void **)&b->h_ring, sizeof(BCtr) * kBRing, hipHostMallocDefault) != hipSuccess)
+            return fail(PTG_ERR_OUT_OF_MEMORY, "pinned counter ring copy");
         if (!b->dist && (rc = galloc(g, (void **)&b->dist, 4 * V * kBatchMax))) return rc;
         const uint32_t chunk = g->cfg.batch_chunk ? g->cfg.batch_chunk : kBatchChunk;
         if (!b->ch_v || b->chunk != chunk) {
@@ -678,6 +840,29 @@ static int ensure_batch(ptg_graph *g, bool sssp) {
             }
             b->nch = (uint32_t)cv.size();
         }
+    }
+    return PTG_OK;
+}
+
+// Learns which rounds of the next batch pull: round r did when round r of
+// this batch expanded at least E x batch_pull edges (munf of entry r + 1).
+// Rounds past the counter ring's length keep pushing.
+static int learn_pull_rounds(ptg_graph *g, uint32_t rounds) {
+    BatchState *b = g->batch;
+    PTG_HIP(hipMemcpyAsync(b->h_ring, b->ring, sizeof(BCtr) * kBRing, hipMemcpyDeviceToHost, g->stream));
+    PTG_HIP(hipStreamSynchronize(g->stream));
+    const float f = g->cfg.batch_pull > 0 ? g->cfg.batch_pull : kBatchPull;
+    const double dense = (double)g->E * f;
+    uint8_t m[kBRing] = {};
+    if (rounds + 2 < kBRing)
+        for (uint32_t r = 0; r < rounds; r++) {
+            uint64_t vis = 0;
+            for (uint32_t i = 0; i < kShards; i++) vis += b->h_ring[r + 1].munf[i];
+            m[r] = (double)vis >= dense ? 1 : 0;
+        }
+    if (memcmp(m, b->h_pmode, kBRing) != 0) {
+        memcpy(b->h_pmode, m, kBRing);
+        return copy_sync(g, b->pmode, b->h_pmode, kBRing, hipMemcpyHostToDevice);
     }
     return PTG_OK;
 }
@@ -710,6 +895,12 @@ static BatchEnv make_batch_env(ptg_graph *g, int sym, bool sssp) {
     e.symmetric = sym;
     e.grid = g->cfg.batch_grid ? g->cfg.batch_grid : 4 * g->grid;
     if (sssp && g->cfg.batch_sssp_grid) e.grid = g->cfg.batch_sssp_grid;
+    if (sssp) {
+        e.sentinel = g->cfg.sssp_sentinel;
+        e.pmode = b->pmode;
+        const float f = g->cfg.batch_pull > 0 ? g->cfg.batch_pull : kBatchPull;
+        e.pull_prev = (unsigned long long)((double)g->E * f / 8.0);
+    }
     return e;
 }
 
@@ -768,7 +959,10 @@ static int run_batch(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, 
     if ((rc = check_symmetric(g, &sym))) return rc;
     if ((rc = ensure_batch(g, sssp))) return rc;
     BatchState *b = g->batch;
-    const BatchEnv env = make_batch_env(g, sym, sssp);
+    const uint2 *cwin = nullptr;
+    if (sssp && g->cfg.batch_pull >= 0 && (rc = sssp_in_pairs(g, &cwin))) return rc;
+    BatchEnv env = make_batch_env(g, sym, sssp);
+    env.cwin = cwin;
     hipGraphExec_t &exec = sssp ? b->sssp_exec : b->bfs_exec;
     BatchEnv &cap_env = sssp ? b->env_sssp : b->env_bfs;
     if (exec && !same_env(env, cap_env)) {
@@ -779,7 +973,10 @@ static int run_batch(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, 
         const uint32_t grid = env.grid;
         rc = capture(g, &exec, [&]() -> int {
             if (sssp) {
-                for (uint32_t j = 0; j < kSsspReplay; j++) k_mssp_round<<<grid, kBlock, 0, g->stream>>>(env, j);
+                for (uint32_t j = 0; j < kSsspReplay; j++) {
+                    k_mssp_round<<<grid, kBlock, 0, g->stream>>>(env, j);
+                    if (env.cwin) k_mssp_pull<<<grid, kBlock, 0, g->stream>>>(env, j);
+                }
                 k_batch_advance<<<1, 64, 0, g->stream>>>(env.p, kSsspReplay);
             } else {
                 for (uint32_t j = 0; j < kBfsReplay; j++) {
@@ -845,6 +1042,7 @@ static int run_batch(ptg_graph *g, bool sssp, const uint32_t *srcs, uint32_t n, 
         PTG_HIP(hipMemcpyAsync(b->h_p, b->p, sizeof(BParams), hipMemcpyDeviceToHost, g->stream));
         PTG_HIP(hipStreamSynchronize(g->stream));
         visits += b->h_p->visits;
+        if (env.cwin && (rc = learn_pull_rounds(g, rounds))) return rc;
     }
     BCtr *accc = b->ring + (rounds + 3) % kBRing;  // an entry no live round uses
     const uint32_t *perm = g->relabeled ? g->d_perm : nullptr;
@@ -908,6 +1106,8 @@ uint64_t batch_state_need(const ptg_graph *g, bool sssp) {
         if (!b || !b->dist) need += al(4 * V * kBatchMax);
         const uint32_t chunk = g->cfg.batch_chunk ? g->cfg.batch_chunk : kBatchChunk;
         if (!b || !b->ch_v || b->chunk != chunk) need += 2 * al(4 * (V + g->E / chunk + 1));
+        if (!b || !b->pmode) need += al(kBRing);
+        if (g->cfg.batch_pull >= 0) need += sssp_in_pairs_need(g);
     }
     return need;
 }
@@ -931,6 +1131,8 @@ void destroy_batch(ptg_graph *g) {
     if (b->h_ctr) (void)hipHostFree(b->h_ctr);
     if (b->h_srcs) (void)hipHostFree(b->h_srcs);
     if (b->h_p) (void)hipHostFree(b->h_p);
+    if (b->h_ring) (void)hipHostFree(b->h_ring);
+    gfree(g, b->pmode);
     delete b;
     g->batch = nullptr;
 }

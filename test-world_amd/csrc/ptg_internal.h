@@ -270,6 +270,8 @@ int check_symmetric(ptg_graph *g, int *sym);
 // Whether the weighted multiset of (v, u, w) equals that of (u, v, w), from
 // the graph's (column, weight) pairs cw (once per graph).
 int check_weight_symmetric(ptg_graph *g, const uint2 *cw, int *wsym);
+int sssp_in_pairs(ptg_graph *g, const uint2 **cwin);
+uint64_t sssp_in_pairs_need(const ptg_graph *g);
 int ensure_in_edges(ptg_graph *g);
 int reach_stats(ptg_graph *g, const uint32_t *val, uint32_t unreached, uint32_t *reached,
                 uint64_t *edges);

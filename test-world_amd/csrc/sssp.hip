@@ -856,6 +856,36 @@ static int ensure_pull(ptg_graph *g) {
     return rc;
 }
 
+// The (source, weight) in-edge pairs of a symmetric weighted graph, in the
+// out-CSR's offsets, for pull rounds of batched SSSP: the out-pairs when
+// every edge's reverse carries the same weight, else the built in-pairs.
+// *cwin stays null when a pull is not possible: the graph is not symmetric,
+// or a path plus an edge could reach the sentinel (a candidate could wrap).
+int sssp_in_pairs(ptg_graph *g, const uint2 **cwin) {
+    *cwin = nullptr;
+    if (!g->weighted) return PTG_OK;
+    int rc, sym = 0, wsym = 0;
+    if ((rc = check_symmetric(g, &sym))) return rc;
+    if (!sym) return PTG_OK;
+    if ((rc = ensure_packed(g)) || (rc = ensure_defer(g))) return rc;  // ensure_defer learns wmax
+    if (!pull_in_range(g)) return PTG_OK;
+    if ((rc = check_weight_symmetric(g, g->d_cw, &wsym))) return rc;
+    if (!wsym && (rc = ensure_pull(g))) return rc;
+    *cwin = wsym ? g->d_cw : g->d_cwin;
+    return PTG_OK;
+}
+
+// Device bytes sssp_in_pairs would still allocate (an upper bound: the
+// in-pairs are counted unless the weights are known to be symmetric).
+uint64_t sssp_in_pairs_need(const ptg_graph *g) {
+    if (!g->weighted || g->symmetric == 0) return 0;
+    uint64_t need = 0;
+    if (!g->d_cw) need += 8ull * std::max<uint64_t>(g->E, 1);
+    if (!g->d_defer) need += 4ull * g->nwords;
+    if (g->wsymmetric != 1 && !g->d_cwin) need += 8ull * std::max<uint64_t>(g->E, 1) + 4ull * g->V + 4;
+    return need;
+}
+
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     int rc;
     LaunchTimer timer(g);

@@ -79,6 +79,7 @@ class Config(ctypes.Structure):
         ("stream_pinned", ctypes.c_uint32),
         ("batch_sssp_grid", ctypes.c_uint32),
         ("sssp_pull", ctypes.c_float),
+        ("batch_pull", ctypes.c_float),
     ]
 
 

@@ -115,6 +115,58 @@ def test_batch_rmat16_vs_single_and_replays():
             buf.free()
 
 
+def symmetric_unequal_weights(csr, seed):
+    """The graph's structure with an independent random weight on each
+    direction of every edge (in-pairs then differ from the out-pairs)."""
+    ptr, col = csr.node_pointers, csr.cols
+    w = np.random.default_rng(seed).integers(1, 65, col.size).astype(np.uint32)
+    return ptr, col, w
+
+
+@pytest.mark.parametrize("pull", [-1.0, 0.0, 1e-9], ids=["never", "auto", "every_round"])
+@pytest.mark.parametrize("weights", ["same_both_ways", "differ_by_direction"])
+def test_batch_sssp_pull_rounds(pull, weights):
+    """Batched SSSP pull rounds: the first batch on a graph pushes and
+    records which rounds were dense; later batches pull in those rounds
+    (every round but the first with batch_pull = 1e-9). Rows of three batches
+    in a row (different sources, then a smaller batch) equal the single
+    queries and, for three rows each, the oracle."""
+    csr = pg.gen_rmat(15, 16, seed=21).add_weights(1, 64, seed=22)
+    if weights == "same_both_ways":
+        ptr, col, w = csr.node_pointers, csr.cols, csr.weights
+    else:
+        ptr, col, w = symmetric_unequal_weights(csr, 23)
+    deg = np.diff(np.append(ptr.astype(np.int64), col.size))
+    rng = np.random.default_rng(24)
+    with pg.Graph(ptr, col, w, config=pg.default_config(batch_pull=pull)) as g:
+        assert g.is_symmetric()
+        for b, n in enumerate((64, 64, 23)):
+            srcs = rng.choice(np.nonzero(deg > 0)[0], n, replace=False).astype(np.uint32)
+            say("case batch sssp pull=%g weights=%s batch %d n=%d" % (pull, weights, b, n))
+            dist, st = g.sssp_batch(srcs)
+            for k in range(n):
+                np.testing.assert_array_equal(dist[k], g.sssp(int(srcs[k]))[0], err_msg="row %d" % k)
+            for k in (0, n // 2, n - 1):
+                np.testing.assert_array_equal(dist[k], oracle.dijkstra(ptr, col, w, int(srcs[k])),
+                                              err_msg="oracle row %d" % k)
+            assert st.reached == int((dist != oracle.SSSP_SENTINEL_SHARE).sum())
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=os.path.basename)
+def test_batch_fixture_pull_every_round(path):
+    """Every fixture (directed ones push only) with pulls in every round after
+    the first: two batches, the second equal to the oracle."""
+    f = load(path)
+    ptr, col, w = f["ptr"], f["col"], f["w"]
+    srcs = batch_sources(ptr.size, np.random.default_rng(7))
+    say("case batch pull %s V=%d" % (os.path.basename(path), ptr.size))
+    with pg.Graph(ptr, col, w, config=pg.default_config(batch_pull=1e-9)) as g:
+        first, _ = g.sssp_batch(srcs)
+        dist, _ = g.sssp_batch(srcs)
+        np.testing.assert_array_equal(dist, first)
+        check_rows(ptr, col, w, srcs, g.bfs_batch(srcs)[0], dist)
+
+
 def random_directed(rng, V, E, wmax):
     src = rng.integers(0, V, E)
     dst = rng.integers(0, V, E)

@@ -179,7 +179,7 @@ def batched_scale24_leg(args, pg, np, cores, log):
             tuned = None
             if not args.no_tune:
                 c = g.autotune(srcs, what=pg.TUNE_BFS_BATCH | pg.TUNE_SSSP_BATCH)
-                tuned = {f: getattr(c, f) for f in ("batch_td_to_bu", "batch_grid", "batch_sssp_grid", "batch_delta", "batch_chunk")}
+                tuned = {f: getattr(c, f) for f in ("batch_td_to_bu", "batch_grid", "batch_sssp_grid", "batch_delta", "batch_chunk", "batch_pull")}
             r = batched_leg(g, srcs, V, pg, log, name)
             r.update({"V": V, "E": E, "tuning": tuned})
             out[name] = r
@@ -507,7 +507,7 @@ def main():
     if not args.no_tune and not args.no_batch:
         t = time.time()
         c = g.autotune(srcs64, what=pg.TUNE_BFS_BATCH | pg.TUNE_SSSP_BATCH)
-        tuned = {f: getattr(c, f) for f in ("batch_td_to_bu", "batch_grid", "batch_sssp_grid", "batch_delta", "batch_chunk")}
+        tuned = {f: getattr(c, f) for f in ("batch_td_to_bu", "batch_grid", "batch_sssp_grid", "batch_delta", "batch_chunk", "batch_pull")}
         log("autotuner %.1fs: %s" % (time.time() - t, tuned))
     batched = None
     if not args.no_batch:

@@ -267,7 +267,7 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
 #define PTG_TUNE_BFS 0x1u         /* td_to_bu, bu_to_td */
 #define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
-#define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_sssp_grid, batch_chunk */
+#define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_sssp_grid, batch_chunk, batch_pull */
 #define PTG_TUNE_STREAM 0x10u     /* streamed graphs only: chunk_bytes, stream_pinned, timed on BFS from
                                      the sources plus 2 PageRank iterations (symmetric graphs) */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);

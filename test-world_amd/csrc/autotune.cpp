@@ -148,6 +148,7 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     if (!c.batch_sssp_grid) c.batch_sssp_grid = c.batch_grid;
     if (!c.batch_delta) c.batch_delta = 256;
     if (!c.batch_chunk) c.batch_chunk = 128;
+    if (c.batch_pull == 0) c.batch_pull = 0.25f;  // the batched SSSP default
     Tuner t{g, {}, {}};
     for (int i = 0; i < n && i < 4; i++) t.plain.push_back(g->relabeled ? g->h_perm[srcs[i]] : srcs[i]);
     for (int i = 0; i < n && i < 64; i++) t.batch.push_back(srcs[i]);
@@ -175,6 +176,9 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kSsspBatch, &c.batch_delta, std::vector<uint32_t>{32, 64, 128, 256, 1024, 4096});
         if (!rc) rc = t.knob(kSsspBatch, &c.batch_sssp_grid, std::vector<uint32_t>{G, 2 * G, 4 * G, 8 * G, 16 * G, 32 * G, 64 * G});
         if (!rc) rc = t.knob(kSsspBatch, &c.batch_chunk, std::vector<uint32_t>{64, 128, 256, 512, 1024});
+        // a value's first batch learns its pull rounds, the second (timed
+        // too; the faster counts) uses them; -1 = push only
+        if (!rc) rc = t.knob(kSsspBatch, &c.batch_pull, std::vector<float>{0.25f, 0.1f, 0.5f, -1});
     }
     if (best) *best = c;
     return rc;

@@ -1,4 +1,4 @@
-// PageRank for gfx950: pull over in-edges, FP32 with compensated sums.
+// PageRank for gfx950: pull over in-edges, FP32 ranks from exact sums.
 //
 // The reference has no PageRank arithmetic (prCaculateInShare,
 // ptGraph/main.cu:891-959, launches nothing; testPagerank runs CC at
@@ -6,13 +6,21 @@
 //   r0(v) = 1/V
 //   r'(v) = (1-d)/V + d * ( sum_{u->v} r(u)/outdeg(u) + D/V ),
 //   D     = sum of r(u) over vertices with outdeg(u) = 0.
-// Per iteration, a gather kernel over static in-degree bins (hub lists split
-// into chunks) and a combine kernel that finishes the hubs; the next gather
-// sums the dangling mass from both kernels' per-workgroup partials. Kernel
-// boundaries order all of it, so no inter-workgroup fences or counters are
-// needed. Every sum is a (hi, lo) pair updated with
-// Knuth's TwoSum, and every reduction runs in a fixed order, so results are
-// bit-identical run to run.
+//
+// Sums. Every sum of contributions, and the dangling mass, is an unsigned
+// 64-bit fixed-point number with 63 fraction bits (Q1.63): each FP32 term in
+// [0, 1] is truncated to a multiple of 2^-63 and added exactly. A sum is then
+// independent of the order and grouping of its terms, and its error is below
+// 2^-63 per term. Any split of the work (grid size, hub chunks, the
+// allocation-free path's buffers) gives the same bits, run after run.
+// The rank is rounded once: r = base + d * sum in FP64 from the exact sum,
+// then to FP32.
+//
+// Work. Per iteration, a gather kernel over static in-degree bins (hub lists
+// cut into chunks) and a combine kernel that finishes the hubs; the next
+// gather adds up the dangling mass from both kernels' per-workgroup
+// partials. Kernel boundaries order all of it, so no inter-workgroup fences
+// or counters are needed.
 #include <string.h>
 
 #include <algorithm>
@@ -24,42 +32,28 @@
 
 namespace ptg {
 
-struct CF {
-    float hi, lo;
-};
+constexpr uint32_t kPrHubLds = 5088;  // hub contributions staged in LDS (20352 B: 8 workgroups per CU fit)
 
-__device__ __forceinline__ void cf_add(CF &a, float x) {
-    const float s = a.hi + x;
-    const float bb = s - a.hi;
-    const float err = (a.hi - (s - bb)) + (x - bb);
-    a.hi = s;
-    a.lo += err;
+// x in [0, 1] as Q1.63, truncated (zero and subnormals give 0).
+__device__ __forceinline__ unsigned long long fx63(float x) {
+    const uint32_t b = __float_as_uint(x);
+    const int e = (int)((b >> 23) & 0xFFu);
+    if (e == 0) return 0;
+    const unsigned long long m = (b & 0x7FFFFFu) | 0x800000u;
+    const int sh = e - 87;  // x = m * 2^(e - 150), so x * 2^63 = m * 2^(e - 87); e <= 127
+    return sh >= 0 ? m << sh : (sh > -64 ? m >> -sh : 0ull);
 }
 
-__device__ __forceinline__ CF cf_merge(CF a, CF b) {
-    const float s = a.hi + b.hi;
-    const float bb = s - a.hi;
-    const float err = (a.hi - (s - bb)) + (b.hi - bb);
-    return CF{s, (a.lo + b.lo) + err};
-}
+__device__ __forceinline__ double fx63_to_f64(unsigned long long s) { return (double)s * 0x1p-63; }
 
-__device__ __forceinline__ CF wave_cf_sum(CF a) {
-    for (int d = 32; d >= 1; d >>= 1) {
-        CF b;
-        b.hi = __shfl_xor(a.hi, d, 64);
-        b.lo = __shfl_xor(a.lo, d, 64);
-        a = cf_merge(a, b);
-    }
-    return a;
-}
-
-// Sum over the workgroup in a fixed order; every thread gets the result.
-__device__ __forceinline__ CF block_cf_sum(CF a, CF *lds) {
-    a = wave_cf_sum(a);
-    if (lane_id() == 0) lds[threadIdx.x >> 6] = a;
+// Sum over the workgroup (Waves waves); every thread gets the result.
+template <uint32_t Waves>
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long x, unsigned long long *lds) {
+    x = wave_sum_u64(x);
+    if (lane_id() == 0) lds[threadIdx.x >> 6] = x;
     __syncthreads();
-    CF t = lds[0];
-    for (int w = 1; w < kWavesPerBlock; w++) t = cf_merge(t, lds[w]);
+    unsigned long long t = 0;
+    for (uint32_t w = 0; w < Waves; w++) t += lds[w];
     __syncthreads();
     return t;
 }
@@ -73,12 +67,13 @@ struct PrArgs {
     const float *cin;               // r(u)/outdeg(u) of the previous iteration
     float *cout;                    // same for this iteration
     float *rank_out;                // ranks (last iteration only) or null
-    const float *d0;                // (hi, lo) of the dangling mass D for iteration 0, or null
-    const float *part_in;           // previous iteration's dangling partials: gather workgroups
-    const float *cpart_in;          //   and combine workgroups, (hi, lo) each
-    float *part_out, *cpart_out;    // this iteration's partials
+    bool first;                     // iteration 0: the dangling mass is d0
+    double d0;                      // D of iteration 0 (dangling vertices / V)
+    const unsigned long long *part_in;   // previous iteration's dangling partials: gather workgroups
+    const unsigned long long *cpart_in;  //   and combine workgroups
+    unsigned long long *part_out, *cpart_out;  // this iteration's partials
     float *base_out;                // this iteration's base value, for the combine kernel
-    float *lpart;                   // hub chunk partials (hi, lo)
+    unsigned long long *lpart;      // hub chunk sums
     uint32_t gather_blocks, combine_blocks;
     uint32_t n_vert;                // V
     float damping, inv_v;
@@ -87,55 +82,48 @@ struct PrArgs {
 struct PrVertex {
     const PrArgs &a;
     float base;
-    CF &dang;
-    // r = base + d * sum; contrib = r / outdeg (0 for a dangling vertex,
-    // whose r joins the dangling mass instead).
-    __device__ __forceinline__ void finalize(uint32_t v, float sum) const {
-        const float r = base + a.damping * sum;
+    unsigned long long &dang;
+    // r = base + d * sum (rounded once); contrib = r / outdeg (0 for a
+    // dangling vertex, whose r joins the dangling mass instead).
+    __device__ __forceinline__ void finalize(uint32_t v, unsigned long long sum) const {
+        const float r = (float)((double)base + (double)a.damping * fx63_to_f64(sum));
         const uint32_t od = a.ooff[v + 1] - a.ooff[v];
         a.cout[v] = od ? r / (float)od : 0.0f;
         if (a.rank_out) a.rank_out[v] = r;
-        if (!od) cf_add(dang, r);
+        if (!od) dang += fx63(r);
     }
 };
 
-// base = (1-d)/V + d * D/V, with D summed from the previous iteration's
-// partials in the same fixed order by every workgroup.
-__device__ __forceinline__ float pr_base(const PrArgs &a, CF *lds) {
-    float dang;
-    if (a.d0) {
-        dang = a.d0[0] + a.d0[1];
+// base = (1-d)/V + d * D/V, D summed from the previous iteration's partials.
+template <uint32_t TB>
+__device__ __forceinline__ float pr_base(const PrArgs &a, unsigned long long *lds) {
+    double dang;
+    if (a.first) {
+        dang = a.d0;
     } else {
-        CF t{0.0f, 0.0f};
-        for (uint32_t j = threadIdx.x; j < a.gather_blocks; j += kBlock)
-            t = cf_merge(t, CF{a.part_in[2 * j], a.part_in[2 * j + 1]});
-        for (uint32_t j = threadIdx.x; j < a.combine_blocks; j += kBlock)
-            t = cf_merge(t, CF{a.cpart_in[2 * j], a.cpart_in[2 * j + 1]});
-        t = block_cf_sum(t, lds);
-        dang = t.hi + t.lo;
+        unsigned long long t = 0;
+        for (uint32_t j = threadIdx.x; j < a.gather_blocks; j += TB) t += a.part_in[j];
+        for (uint32_t j = threadIdx.x; j < a.combine_blocks; j += TB) t += a.cpart_in[j];
+        dang = fx63_to_f64(block_sum<TB / 64>(t, lds));
     }
-    return (1.0f - a.damping) * a.inv_v + a.damping * (dang * a.inv_v);
+    return (float)((1.0 - (double)a.damping) * (double)a.inv_v + (double)a.damping * dang * (double)a.inv_v);
 }
 
-constexpr uint32_t kPrHubLds = 5088;  // 20352 B: 8 workgroups of the grid per CU still fit
-
-// Gathers every vertex's in-neighbour contributions. Hub lists are cut into
-// chunks whose partial sums go to lpart; k_pr_combine finishes those
-// vertices. Each workgroup leaves its share of the dangling mass in part_out.
+// Gathers every vertex's in-neighbour contributions; the contributions of
+// the first kPrHubLds vertices (the hubs, in the default degree order: about
+// a quarter of the in-list entries at R-MAT scale 22) come from LDS. Hub
+// lists are cut into chunks whose sums go to lpart; k_pr_combine finishes
+// those vertices. Each workgroup leaves its share of the dangling mass in
+// part_out.
 __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
-    __shared__ CF lds[kWavesPerBlock];
-    // The contributions of the first kPrHubLds vertices are staged in LDS:
-    // in the default degree order these are the hubs, about a quarter of the
-    // in-list entries at R-MAT scale 22. 20 KB leaves room for the 8
-    // workgroups per CU of the grid; 32 KB (5 per CU) was 11% slower, and
-    // 128 KB in 1024-thread workgroups (1 per CU) 20% slower.
+    __shared__ unsigned long long lds[kWavesPerBlock];
     __shared__ float s_hub[kPrHubLds];
     const uint32_t nh = min(kPrHubLds, a.n_vert);
     for (uint32_t i = threadIdx.x; i < nh; i += kBlock) s_hub[i] = a.cin[i];
     __syncthreads();
-    auto contrib = [&](uint32_t u) -> float { return u < nh ? s_hub[u] : a.cin[u]; };
-    CF my_dang{0.0f, 0.0f};
-    const float base = pr_base(a, lds);
+    auto term = [&](uint32_t u) -> unsigned long long { return fx63(u < nh ? s_hub[u] : a.cin[u]); };
+    unsigned long long my_dang = 0;
+    const float base = pr_base<kBlock>(a, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.base_out[0] = base;
     const PrVertex pv{a, base, my_dang};
     const uint32_t lane = lane_id();
@@ -153,13 +141,10 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
         const uint32_t v = a.bins.ql[lo];
         const uint64_t b = (uint64_t)a.ioff[v] + (uint64_t)(c - a.bins.lb[lo]) * kChunkEdges;
         const uint64_t e = min((uint64_t)a.ioff[v + 1], b + kChunkEdges);
-        CF acc{0.0f, 0.0f};
-        for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) cf_add(acc, contrib(a.icol[i]));
-        acc = block_cf_sum(acc, lds);
-        if (threadIdx.x == 0) {
-            a.lpart[2ull * c] = acc.hi;
-            a.lpart[2ull * c + 1] = acc.lo;
-        }
+        unsigned long long acc = 0;
+        for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) acc += term(a.icol[i]);
+        acc = block_sum<kWavesPerBlock>(acc, lds);
+        if (threadIdx.x == 0) a.lpart[c] = acc;
     }
 
     // Medium in-degree: one wave per vertex.
@@ -168,11 +153,10 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
     const uint32_t nm = a.bins.ctr->n_m;
     for (uint32_t i = gw; i < nm; i += nw) {
         const uint32_t v = a.bins.qm[i];
-        const uint64_t b = a.ioff[v], e = a.ioff[v + 1];
-        CF acc{0.0f, 0.0f};
-        for (uint64_t k = b + lane; k < e; k += 64) cf_add(acc, contrib(a.icol[k]));
-        acc = wave_cf_sum(acc);
-        if (lane == 0) pv.finalize(v, acc.hi + acc.lo);
+        unsigned long long acc = 0;
+        for (uint64_t k = a.ioff[v] + lane; k < a.ioff[v + 1]; k += 64) acc += term(a.icol[k]);
+        acc = wave_sum_u64(acc);
+        if (lane == 0) pv.finalize(v, acc);
     }
 
     // Small in-degree: one lane per vertex; then in-degree 0.
@@ -181,26 +165,21 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
     const uint32_t ns = a.bins.ctr->n_s;
     for (uint32_t i = gt; i < ns; i += nt) {
         const uint32_t v = a.bins.qs[i];
-        const uint32_t b = a.ioff[v], e = a.ioff[v + 1];
-        CF acc{0.0f, 0.0f};
-        for (uint32_t k = b; k < e; k++) cf_add(acc, contrib(a.icol[k]));
-        pv.finalize(v, acc.hi + acc.lo);
+        unsigned long long acc = 0;
+        for (uint32_t k = a.ioff[v]; k < a.ioff[v + 1]; k++) acc += term(a.icol[k]);
+        pv.finalize(v, acc);
     }
-    for (uint32_t i = gt; i < a.n_zero; i += nt) pv.finalize(a.zero_list[i], 0.0f);
+    for (uint32_t i = gt; i < a.n_zero; i += nt) pv.finalize(a.zero_list[i], 0ull);
 
-    const CF bd = block_cf_sum(my_dang, lds);
-    if (threadIdx.x == 0) {
-        a.part_out[2 * blockIdx.x] = bd.hi;
-        a.part_out[2 * blockIdx.x + 1] = bd.lo;
-    }
+    const unsigned long long bd = block_sum<kWavesPerBlock>(my_dang, lds);
+    if (threadIdx.x == 0) a.part_out[blockIdx.x] = bd;
 }
 
-// Finishes the hub vertices: one wave per hub sums its chunk partials (lane
-// stride, then a fixed butterfly). Each workgroup leaves the dangling mass of
-// its hubs in cpart_out.
+// Finishes the hub vertices: one wave per hub adds its chunk sums. Each
+// workgroup leaves the dangling mass of its hubs in cpart_out.
 __global__ __launch_bounds__(kBlock) void k_pr_combine(PrArgs a) {
-    __shared__ CF lds[kWavesPerBlock];
-    CF my_dang{0.0f, 0.0f};
+    __shared__ unsigned long long lds[kWavesPerBlock];
+    unsigned long long my_dang = 0;
     const PrVertex pv{a, a.base_out[0], my_dang};
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -210,17 +189,13 @@ __global__ __launch_bounds__(kBlock) void k_pr_combine(PrArgs a) {
         const uint32_t v = a.bins.ql[l];
         const uint32_t first = a.bins.lb[l];
         const uint32_t n = bin_chunks(a.ioff[v + 1] - a.ioff[v]);
-        CF t{0.0f, 0.0f};
-        for (uint32_t j = lane; j < n; j += 64)
-            t = cf_merge(t, CF{a.lpart[2ull * (first + j)], a.lpart[2ull * (first + j) + 1]});
-        t = wave_cf_sum(t);
-        if (lane == 0) pv.finalize(v, t.hi + t.lo);
+        unsigned long long t = 0;
+        for (uint32_t j = lane; j < n; j += 64) t += a.lpart[first + j];
+        t = wave_sum_u64(t);
+        if (lane == 0) pv.finalize(v, t);
     }
-    const CF bd = block_cf_sum(my_dang, lds);
-    if (threadIdx.x == 0) {
-        a.cpart_out[2 * blockIdx.x] = bd.hi;
-        a.cpart_out[2 * blockIdx.x + 1] = bd.lo;
-    }
+    const unsigned long long bd = block_sum<kWavesPerBlock>(my_dang, lds);
+    if (threadIdx.x == 0) a.cpart_out[blockIdx.x] = bd;
 }
 
 // r0 = 1/V: contrib0(v) = (1/V)/outdeg(v), or 0 for a dangling vertex.
@@ -230,13 +205,6 @@ __global__ __launch_bounds__(kBlock) void k_pr_init(uint32_t V, const uint32_t *
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
         const uint32_t od = ooff[v + 1] - ooff[v];
         contrib[v] = od ? r0 / (float)od : 0.0f;
-    }
-}
-
-__global__ void k_set_f32x2(float *p, float x, float y) {
-    if (threadIdx.x == 0) {
-        p[0] = x;
-        p[1] = y;
     }
 }
 
@@ -283,30 +251,30 @@ static uint32_t combine_blocks(const ptg_graph *g) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kMaxCombineBlocks, (nl + kWavesPerBlock - 1) / kWavesPerBlock));
 }
 
-// dangling and base partials: 2 iterations x (gather, combine) + base + D0
-static uint64_t pr_part_floats(const ptg_graph *g) { return 2 * (2ull * g->grid + 2ull * kMaxCombineBlocks) + 4; }
+// Partial words: 2 iterations x (gather partials, combine partials; at most
+// grid workgroups each), then the base value (a float in a 64-bit word).
+static uint64_t pr_part_words(const ptg_graph *g) { return 4ull * g->grid + 1; }
 
 static int ensure_pr_state(ptg_graph *g) {
     if (g->d_contrib[0]) return PTG_OK;
     int rc;
     const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
-    // d_part: 2 iterations x (gather partials, combine partials) + base + D0
-    const uint64_t part_floats = pr_part_floats(g);
     if ((rc = galloc(g, (void **)&g->d_contrib[0], sizeof(float) * g->V)) ||
         (rc = galloc(g, (void **)&g->d_contrib[1], sizeof(float) * g->V)) ||
-        (rc = galloc(g, (void **)&g->d_part, sizeof(float) * part_floats)) ||
-        (rc = galloc(g, (void **)&g->d_lpart, sizeof(float) * 2 * (chunks + 1))))
+        (rc = galloc(g, (void **)&g->d_part, 8 * pr_part_words(g))) ||
+        (rc = galloc(g, (void **)&g->d_lpart, 8 * (chunks + 1))))
         return rc;
     g->lchunks_cap = (uint32_t)chunks;
     return PTG_OK;
 }
 
 uint64_t pr_state_need(const ptg_graph *g) {
-    if (g->d_contrib[0]) return 0;
     auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
-    const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
-    uint64_t need = 2 * al(4ull * g->V) + al(4 * (2 * (2ull * g->grid + 2ull * kMaxCombineBlocks) + 4)) +
-                    al(8 * (chunks + 1));
+    uint64_t need = 0;
+    if (!g->d_contrib[0]) {
+        const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
+        need += 2 * al(4ull * g->V) + al(8 * pr_part_words(g)) + al(8 * (chunks + 1));
+    }
     if (!g->have_in) need += al(4ull * (g->V + 1)) + al(4 * g->E) + al(8ull * g->V);  // upper bound: transpose
     return need;
 }
@@ -323,13 +291,12 @@ void release_pr_state(ptg_graph *g) {
 }
 
 // Per-iteration buffers of one PageRank run: contributions (2 x V floats),
-// dangling and base partials, hub chunk partials.
+// dangling partials and the base value, hub chunk sums.
 struct PrBufs {
     float *contrib[2];
-    float *part;
-    float *lpart;
+    unsigned long long *part;
+    unsigned long long *lpart;
 };
-
 
 static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st, const PrBufs &buf,
                          bool timed) {
@@ -349,19 +316,15 @@ static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, p
     } else {
         const uint32_t blocks = std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid);
         k_pr_init<<<blocks, kBlock, 0, g->stream>>>(V, g->out.off, r0, buf.contrib[0]);
-        // partial buffers: [it parity][gather | combine], then base, then D0
-        const uint32_t gb = g->grid, cb = combine_blocks(g);
-        float *part[2] = {buf.part, buf.part + 2 * gb};
-        float *cpart[2] = {buf.part + 4 * gb, buf.part + 4 * gb + 2 * kMaxCombineBlocks};
-        float *base = buf.part + 4 * gb + 4 * kMaxCombineBlocks;
-        float *d0 = base + 2;
-        // D0 = (number of dangling vertices) / V, split into (hi, lo).
-        const double d0v = (double)g->out.bins.n_zero / (double)V;
-        const float d0h = (float)d0v;
-        k_set_f32x2<<<1, 64, 0, g->stream>>>(d0, d0h, (float)(d0v - (double)d0h));
-        PTG_HIP(hipGetLastError());
+        // partial words: [it parity][gather | combine], then the base value
+        const uint32_t gb = g->grid;
+        const uint32_t cb = std::min<uint32_t>(gb, combine_blocks(g));
+        unsigned long long *part[2] = {buf.part, buf.part + gb};
+        unsigned long long *cpart[2] = {buf.part + 2 * gb, buf.part + 3 * gb};
+        float *base = reinterpret_cast<float *>(buf.part + 4 * gb);
         for (int t = 0; t < iters; t++) {
             PrArgs a;
+            memset(&a, 0, sizeof a);
             a.bins = g->in.bins.q.with(g->in.bins.d_ctr);
             a.zero_list = g->in.bins.zero;
             a.n_zero = g->in.bins.n_zero;
@@ -371,7 +334,8 @@ static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, p
             a.cin = buf.contrib[t & 1];
             a.cout = buf.contrib[(t + 1) & 1];
             a.rank_out = (t == iters - 1) ? ranks : nullptr;
-            a.d0 = t == 0 ? d0 : nullptr;
+            a.first = t == 0;
+            a.d0 = (double)g->out.bins.n_zero / (double)V;  // D0 = (dangling vertices) / V
             a.part_in = part[(t + 1) & 1];
             a.cpart_in = cpart[(t + 1) & 1];
             a.part_out = part[t & 1];
@@ -383,7 +347,7 @@ static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, p
             a.n_vert = V;
             a.damping = damping;
             a.inv_v = r0;
-            timer.launch(k_pr_gather, dim3(g->grid), dim3(kBlock), a);
+            timer.launch(k_pr_gather, dim3(gb), dim3(kBlock), a);
             k_pr_combine<<<cb, kBlock, 0, g->stream>>>(a);
             PTG_HIP(hipGetLastError());
         }
@@ -444,7 +408,7 @@ static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
 
 uint64_t pr_scratch_need(const ptg_graph *g) {
     const uint64_t chunks = g->in.bins.h_ctr.l_pack & kLChunkMask;
-    return 2 * al256(4ull * g->V) + al256(4 * pr_part_floats(g)) + al256(8 * (chunks + 1));
+    return 2 * al256(4ull * g->V) + al256(8 * pr_part_words(g)) + al256(8 * (chunks + 1));
 }
 
 int run_pagerank_noalloc(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
@@ -470,9 +434,9 @@ int run_pagerank_noalloc(ptg_graph *g, float damping, int iters, float *ranks, p
     p += al256(4ull * g->V);
     buf.contrib[1] = reinterpret_cast<float *>(p);
     p += al256(4ull * g->V);
-    buf.part = reinterpret_cast<float *>(p);
-    p += al256(4 * pr_part_floats(g));
-    buf.lpart = reinterpret_cast<float *>(p);
+    buf.part = reinterpret_cast<unsigned long long *>(p);
+    p += al256(8 * pr_part_words(g));
+    buf.lpart = reinterpret_cast<unsigned long long *>(p);
     return pagerank_core(g, damping, iters, ranks, st, buf, false);
 }
 

@@ -168,8 +168,8 @@ struct ptg_graph {
 
     // PageRank state.
     float *d_contrib[2] = {};
-    float *d_part = nullptr;            // PageRank dangling partials, base value, D0
-    float *d_lpart = nullptr;           // hub chunk partials (hi, lo)
+    unsigned long long *d_part = nullptr;   // PageRank dangling partials and the base value
+    unsigned long long *d_lpart = nullptr;  // hub chunk sums (fixed point)
     uint32_t lchunks_cap = 0;
 
     // Per-launch timing of the main kernel (PTG_FLAG_TIMING).

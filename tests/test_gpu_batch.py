@@ -252,6 +252,7 @@ def test_autotune_and_set_config():
         for f in ("td_to_bu", "bu_to_td", "batch_td_to_bu", "batch_grid", "batch_sssp_grid", "batch_delta",
                   "batch_chunk"):
             assert getattr(c, f) == getattr(got, f) and getattr(c, f) > 0, f
+        assert c.batch_pull == got.batch_pull and c.batch_pull != 0  # -1 = push only
         for f in ("sssp_delta", "sssp_defer_degree", "sssp_defer_window"):  # 0 = automatic
             assert getattr(c, f) == getattr(got, f), f
         say("case autotune result %s" % {f: getattr(c, f) for f, _ in c._fields_})

@@ -196,6 +196,32 @@ def test_device_outputs_and_determinism(rmat16):
         np.testing.assert_array_equal(c2, oracle.cc_reach_min(g.node_pointers, g.cols))
 
 
+def test_pagerank_bits_do_not_depend_on_the_work_split():
+    """PageRank sums are exact fixed-point sums (63 fraction bits), so the
+    ranks are the same bits however the in-lists are divided: one workgroup,
+    an odd grid, every list cut into 64-edge hub chunks, the default bins, and
+    the allocation-free entry point's buffers. And within 1e-6 L1 of FP64."""
+    csr = pg.gen_rmat(14, 16, seed=9)
+    ptr, col = csr.node_pointers, csr.cols
+    want = oracle.pagerank_f64(ptr, col, 0.85, 20)
+    ref = None
+    for cfg in ({}, {"grid_blocks": 1}, {"grid_blocks": 37}, {"small_degree": 1, "large_degree": 64},
+                {"flags": pg.FLAG_NO_RELABEL}):
+        say("case pagerank split %s" % cfg)
+        with pg.Graph.from_csr(csr, weighted=False, config=pg.default_config(**cfg)) as g:
+            r, _ = g.pagerank(0.85, 20)
+            assert float(np.abs(r.astype(np.float64) - want).sum()) <= PR_L1
+            if ref is None:
+                ref = r
+            np.testing.assert_array_equal(r.view(np.uint32), ref.view(np.uint32), err_msg=str(cfg))
+            buf = pg.DeviceBuffer(4 * g.V)
+            try:
+                g.pagerank_noalloc(buf, 0.85, 20)
+                np.testing.assert_array_equal(buf.to_host(np.uint32, g.V), ref.view(np.uint32), err_msg=str(cfg))
+            finally:
+                buf.free()
+
+
 def random_directed(rng, V, E, wmax):
     src = rng.integers(0, V, E)
     dst = rng.integers(0, V, E)

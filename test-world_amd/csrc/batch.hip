@@ -48,6 +48,7 @@ namespace ptg {
 
 constexpr uint32_t kBatchMax = 64;
 constexpr uint32_t kBRing = 64;        // counter ring entries (> replay length + 2)
+static_assert(sizeof(ptg_graph::batch_pull_rounds) == kBRing, "one learned pull flag per ring entry");
 constexpr uint32_t kBfsReplay = 8;     // BFS levels per graph replay
 constexpr uint32_t kSsspReplay = 16;   // SSSP rounds per graph replay
 constexpr uint32_t kLevelCap = 32;     // BFS levels kept as masks; deeper -> per-source path
@@ -106,8 +107,7 @@ struct BatchState {
     uint32_t *dist = nullptr;
     uint32_t *ch_v = nullptr, *ch_b = nullptr;
     uint32_t nch = 0, chunk = 0;
-    uint8_t *pmode = nullptr;                   // SSSP pull rounds learned from the previous batch
-    uint8_t h_pmode[kBRing] = {};
+    uint8_t *pmode = nullptr;                   // SSSP pull rounds (the graph's batch_pull_rounds)
     BCtr *h_ring = nullptr;                     // pinned copy of the counter ring (pull learning)
     uint32_t *out = nullptr;   // n x V staging for host outputs
     uint32_t *srcs = nullptr;  // device copy of the sources
@@ -811,8 +811,7 @@ static int ensure_batch(ptg_graph *g, bool sssp) {
         if (!b->far && (rc = galloc(g, (void **)&b->far, 8 * V))) return rc;
         if (!b->pmode) {
             if ((rc = galloc(g, (void **)&b->pmode, kBRing))) return rc;
-            memset(b->h_pmode, 0, sizeof b->h_pmode);
-            if ((rc = copy_sync(g, b->pmode, b->h_pmode, kBRing, hipMemcpyHostToDevice))) return rc;
+            if ((rc = copy_sync(g, b->pmode, g->batch_pull_rounds, kBRing, hipMemcpyHostToDevice))) return rc;
         }
         if (!b->h_ring && hipHostMalloc((void **)&b->h_ring, sizeof(BCtr) * kBRing, hipHostMallocDefault) != hipSuccess)
             return fail(PTG_ERR_OUT_OF_MEMORY, "pinned counter ring copy");
@@ -860,9 +859,9 @@ static int learn_pull_rounds(ptg_graph *g, uint32_t rounds) {
             for (uint32_t i = 0; i < kShards; i++) vis += b->h_ring[r + 1].munf[i];
             m[r] = (double)vis >= dense ? 1 : 0;
         }
-    if (memcmp(m, b->h_pmode, kBRing) != 0) {
-        memcpy(b->h_pmode, m, kBRing);
-        return copy_sync(g, b->pmode, b->h_pmode, kBRing, hipMemcpyHostToDevice);
+    if (memcmp(m, g->batch_pull_rounds, kBRing) != 0) {
+        memcpy(g->batch_pull_rounds, m, kBRing);
+        return copy_sync(g, b->pmode, g->batch_pull_rounds, kBRing, hipMemcpyHostToDevice);
     }
     return PTG_OK;
 }

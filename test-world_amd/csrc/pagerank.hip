@@ -159,16 +159,29 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
         if (lane == 0) pv.finalize(v, acc);
     }
 
-    // Small in-degree: one lane per vertex; then in-degree 0.
+    // Small in-degree: eight lanes per vertex (a wave takes 8 vertices), so a
+    // column load touches 8 lines of 8 consecutive columns instead of 64
+    // lines, one per lane; the 8 lanes' sums are added by shuffles.
+    const uint32_t ns = a.bins.ctr->n_s;
+    const uint32_t sub = lane & 7;
+    for (uint32_t i0 = gw * 8; i0 < ns; i0 += nw * 8) {
+        const uint32_t i = i0 + (lane >> 3);
+        uint32_t v = 0;
+        unsigned long long acc = 0;
+        if (i < ns) {
+            v = a.bins.qs[i];
+            for (uint32_t k = a.ioff[v] + sub; k < a.ioff[v + 1]; k += 8) acc += term(a.icol[k]);
+        }
+        for (int d = 4; d >= 1; d >>= 1) {
+            const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)acc, d, 64);
+            const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(acc >> 32), d, 64);
+            acc += (unsigned long long)hi << 32 | lo;
+        }
+        if (sub == 0 && i < ns) pv.finalize(v, acc);
+    }
+    // then in-degree 0
     const uint32_t gt = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t nt = gridDim.x * kBlock;
-    const uint32_t ns = a.bins.ctr->n_s;
-    for (uint32_t i = gt; i < ns; i += nt) {
-        const uint32_t v = a.bins.qs[i];
-        unsigned long long acc = 0;
-        for (uint32_t k = a.ioff[v]; k < a.ioff[v + 1]; k++) acc += term(a.icol[k]);
-        pv.finalize(v, acc);
-    }
     for (uint32_t i = gt; i < a.n_zero; i += nt) pv.finalize(a.zero_list[i], 0ull);
 
     const unsigned long long bd = block_sum<kWavesPerBlock>(my_dang, lds);

@@ -208,6 +208,9 @@ struct ptg_graph {
 
     // Batched multi-source state (batch.hip), allocated on the first batch.
     ptg::BatchState *batch = nullptr;
+    // batched SSSP: the rounds of the next batch that pull, learned from the
+    // last batch (kept when the batch state is released, e.g. by a server)
+    uint8_t batch_pull_rounds[64] = {};
     // PTG_FLAG_STREAM: host-resident edges and the device chunk cache (stream.hip).
     ptg::StreamState *streamed = nullptr;
 

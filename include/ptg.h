@@ -117,6 +117,10 @@ typedef struct ptg_config {
                                      expanded >= E x batch_pull edges and round r - 1 of this one
                                      >= E x batch_pull / 8; the distances are the same.
                                      0 = auto (0.25), < 0 = never */
+    uint32_t pr_block;            /* threads per workgroup of the PageRank gather: 256, 512 or 1024
+                                     (0 = 256). The hub contributions it stages in LDS grow with
+                                     it (5088, 10176, 20352 vertices), so a CU holds the same 160
+                                     KB of LDS; the ranks are the same bits for every value */
 } ptg_config;
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
@@ -208,7 +212,7 @@ PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
 /* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta,
  * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid, batch_sssp_grid,
- * batch_td_to_bu, batch_chunk, batch_pull, stream_pinned, PTG_FLAG_TIMING and, for a
+ * batch_td_to_bu, batch_chunk, batch_pull, pr_block, stream_pinned, PTG_FLAG_TIMING and, for a
  * streamed graph, chunk_bytes (the chunks are cut again and the slot cache
  * resized; PTG_ERR_BUDGET, with the previous size kept, when fewer than two
  * slots would fit). Every other field must equal the graph's
@@ -268,6 +272,7 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
 #define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
 #define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_sssp_grid, batch_chunk, batch_pull */
+#define PTG_TUNE_PAGERANK 0x20u   /* pr_block (timed on 3 iterations) */
 #define PTG_TUNE_STREAM 0x10u     /* streamed graphs only: chunk_bytes, stream_pinned, timed on BFS from
                                      the sources plus 2 PageRank iterations (symmetric graphs) */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);

@@ -20,7 +20,7 @@ namespace ptg {
 
 namespace {
 
-enum Query { kBfs, kSssp, kBfsBatch, kSsspBatch, kStream };
+enum Query { kBfs, kSssp, kBfsBatch, kSsspBatch, kStream, kPagerank };
 
 struct Tuner {
     ptg_graph *g;
@@ -43,6 +43,9 @@ struct Tuner {
                 if ((rc = run_pagerank_streamed(g, 0.85f, 2, reinterpret_cast<float *>(g->d_val), &st))) return rc;
                 t += (double)st.kernel_ns;
             }
+        } else if (q == kPagerank) {
+            if ((rc = run_pagerank(g, 0.85f, 3, reinterpret_cast<float *>(g->d_val), &st))) return rc;
+            t = (double)st.kernel_ns;
         } else if (q == kBfs || q == kSssp) {
             for (uint32_t s : plain) {
                 rc = q == kBfs ? run_bfs(g, s, g->d_val, &st) : run_sssp(g, s, g->d_val, &st);
@@ -171,6 +174,11 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
     if (what & PTG_TUNE_BFS_BATCH) {
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_td_to_bu, std::vector<float>{2, 6, 14, 30});
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G, 8 * G});
+    }
+    if (what & PTG_TUNE_PAGERANK) {
+        // threads per gather workgroup (the LDS hub stage grows with it)
+        if (!c.pr_block) c.pr_block = 256;
+        if (!rc) rc = t.knob(kPagerank, &c.pr_block, std::vector<uint32_t>{256, 512, 1024});
     }
     if ((what & PTG_TUNE_SSSP_BATCH) && g->weighted) {
         if (!rc) rc = t.knob(kSsspBatch, &c.batch_delta, std::vector<uint32_t>{32, 64, 128, 256, 1024, 4096});

@@ -32,7 +32,9 @@
 
 namespace ptg {
 
-constexpr uint32_t kPrHubLds = 5088;  // hub contributions staged in LDS (20352 B: 8 workgroups per CU fit)
+// Hub contributions staged in LDS per 256 threads of a gather workgroup:
+// 20352 B, so 8 workgroups of 256 threads (or 4 of 512, 2 of 1024) fit a CU.
+constexpr uint32_t kPrHubLds = 5088;
 
 // x in [0, 1] as Q1.63, truncated (zero and subnormals give 0).
 __device__ __forceinline__ unsigned long long fx63(float x) {
@@ -109,21 +111,23 @@ __device__ __forceinline__ float pr_base(const PrArgs &a, unsigned long long *ld
     return (float)((1.0 - (double)a.damping) * (double)a.inv_v + (double)a.damping * dang * (double)a.inv_v);
 }
 
-// Gathers every vertex's in-neighbour contributions; the contributions of
-// the first kPrHubLds vertices (the hubs, in the default degree order: about
-// a quarter of the in-list entries at R-MAT scale 22) come from LDS. Hub
-// lists are cut into chunks whose sums go to lpart; k_pr_combine finishes
-// those vertices. Each workgroup leaves its share of the dangling mass in
-// part_out.
-__global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
-    __shared__ unsigned long long lds[kWavesPerBlock];
-    __shared__ float s_hub[kPrHubLds];
-    const uint32_t nh = min(kPrHubLds, a.n_vert);
-    for (uint32_t i = threadIdx.x; i < nh; i += kBlock) s_hub[i] = a.cin[i];
+// Gathers every vertex's in-neighbour contributions with TB threads per
+// workgroup; the contributions of the first NH vertices (the hubs, in the
+// default degree order: about a quarter of the in-list entries at R-MAT scale
+// 22 for 5088 of them) come from LDS. Hub lists are cut into chunks whose sums
+// go to lpart; k_pr_combine finishes those vertices. Each workgroup leaves its
+// share of the dangling mass in part_out.
+template <uint32_t TB, uint32_t NH>
+__global__ __launch_bounds__(TB) void k_pr_gather(PrArgs a) {
+    constexpr uint32_t kWaves = TB / 64;
+    __shared__ unsigned long long lds[kWaves];
+    __shared__ float s_hub[NH];
+    const uint32_t nh = min(NH, a.n_vert);
+    for (uint32_t i = threadIdx.x; i < nh; i += TB) s_hub[i] = a.cin[i];
     __syncthreads();
     auto term = [&](uint32_t u) -> unsigned long long { return fx63(u < nh ? s_hub[u] : a.cin[u]); };
     unsigned long long my_dang = 0;
-    const float base = pr_base<kBlock>(a, lds);
+    const float base = pr_base<TB>(a, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.base_out[0] = base;
     const PrVertex pv{a, base, my_dang};
     const uint32_t lane = lane_id();
@@ -142,14 +146,14 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
         const uint64_t b = (uint64_t)a.ioff[v] + (uint64_t)(c - a.bins.lb[lo]) * kChunkEdges;
         const uint64_t e = min((uint64_t)a.ioff[v + 1], b + kChunkEdges);
         unsigned long long acc = 0;
-        for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) acc += term(a.icol[i]);
-        acc = block_sum<kWavesPerBlock>(acc, lds);
+        for (uint64_t i = b + threadIdx.x; i < e; i += TB) acc += term(a.icol[i]);
+        acc = block_sum<kWaves>(acc, lds);
         if (threadIdx.x == 0) a.lpart[c] = acc;
     }
 
     // Medium in-degree: one wave per vertex.
-    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kWaves;
     const uint32_t nm = a.bins.ctr->n_m;
     for (uint32_t i = gw; i < nm; i += nw) {
         const uint32_t v = a.bins.qm[i];
@@ -180,11 +184,11 @@ __global__ __launch_bounds__(kBlock) void k_pr_gather(PrArgs a) {
         if (sub == 0 && i < ns) pv.finalize(v, acc);
     }
     // then in-degree 0
-    const uint32_t gt = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t nt = gridDim.x * kBlock;
+    const uint32_t gt = blockIdx.x * TB + threadIdx.x;
+    const uint32_t nt = gridDim.x * TB;
     for (uint32_t i = gt; i < a.n_zero; i += nt) pv.finalize(a.zero_list[i], 0ull);
 
-    const unsigned long long bd = block_sum<kWavesPerBlock>(my_dang, lds);
+    const unsigned long long bd = block_sum<kWaves>(my_dang, lds);
     if (threadIdx.x == 0) a.part_out[blockIdx.x] = bd;
 }
 
@@ -329,8 +333,10 @@ static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, p
     } else {
         const uint32_t blocks = std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid);
         k_pr_init<<<blocks, kBlock, 0, g->stream>>>(V, g->out.off, r0, buf.contrib[0]);
-        // partial words: [it parity][gather | combine], then the base value
-        const uint32_t gb = g->grid;
+        // partial words: [it parity][gather | combine], then the base value;
+        // the gather runs the grid's threads in workgroups of pr_block
+        const uint32_t tb = g->cfg.pr_block ? g->cfg.pr_block : 256u;
+        const uint32_t gb = std::max<uint32_t>(1, g->grid * 256u / tb);
         const uint32_t cb = std::min<uint32_t>(gb, combine_blocks(g));
         unsigned long long *part[2] = {buf.part, buf.part + gb};
         unsigned long long *cpart[2] = {buf.part + 2 * gb, buf.part + 3 * gb};
@@ -360,7 +366,12 @@ static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, p
             a.n_vert = V;
             a.damping = damping;
             a.inv_v = r0;
-            timer.launch(k_pr_gather, dim3(gb), dim3(kBlock), a);
+            if (tb == 1024)
+                timer.launch(k_pr_gather<1024, 4 * kPrHubLds>, dim3(gb), dim3(1024), a);
+            else if (tb == 512)
+                timer.launch(k_pr_gather<512, 2 * kPrHubLds>, dim3(gb), dim3(512), a);
+            else
+                timer.launch(k_pr_gather<256, kPrHubLds>, dim3(gb), dim3(256), a);
             k_pr_combine<<<cb, kBlock, 0, g->stream>>>(a);
             PTG_HIP(hipGetLastError());
         }

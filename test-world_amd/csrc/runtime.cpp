@@ -242,6 +242,8 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     if (c.sssp_sentinel == 0) c.sssp_sentinel = 0xFFFFFFFEu;
     if (c.sssp_pull != c.sssp_pull || c.batch_pull != c.batch_pull)
         return fail(PTG_ERR_INVALID_ARG, "graph_create: NaN sssp_pull or batch_pull");
+    if (c.pr_block && c.pr_block != 256 && c.pr_block != 512 && c.pr_block != 1024)
+        return fail(PTG_ERR_INVALID_ARG, "graph_create: pr_block is 0, 256, 512 or 1024");
     if (!(c.flags & PTG_FLAG_NO_VALIDATE)) {
         int rc = ptg_validate_csr(V, E, ptr, cols, wedges);
         if (rc) return rc;
@@ -583,6 +585,8 @@ int ptg_graph_set_config(ptg_graph *g, const ptg_config *c) {
         return fail(PTG_ERR_INVALID_ARG, "negative or NaN switch threshold");
     if (c->sssp_pull != c->sssp_pull || c->batch_pull != c->batch_pull)
         return fail(PTG_ERR_INVALID_ARG, "NaN sssp_pull or batch_pull");
+    if (c->pr_block && c->pr_block != 256 && c->pr_block != 512 && c->pr_block != 1024)
+        return fail(PTG_ERR_INVALID_ARG, "pr_block is 0, 256, 512 or 1024");
     const bool replan = g->streamed && c->chunk_bytes != o.chunk_bytes;
     g->cfg = *c;
     g->pull_hist[0] = g->pull_hist[1] = ~0ull;  // SSSP pull rounds are looked for again

@@ -206,7 +206,8 @@ def test_pagerank_bits_do_not_depend_on_the_work_split():
     want = oracle.pagerank_f64(ptr, col, 0.85, 20)
     ref = None
     for cfg in ({}, {"grid_blocks": 1}, {"grid_blocks": 37}, {"small_degree": 1, "large_degree": 64},
-                {"flags": pg.FLAG_NO_RELABEL}):
+                {"flags": pg.FLAG_NO_RELABEL}, {"pr_block": 512}, {"pr_block": 1024},
+                {"pr_block": 1024, "grid_blocks": 3}):
         say("case pagerank split %s" % cfg)
         with pg.Graph.from_csr(csr, weighted=False, config=pg.default_config(**cfg)) as g:
             r, _ = g.pagerank(0.85, 20)
@@ -220,6 +221,23 @@ def test_pagerank_bits_do_not_depend_on_the_work_split():
                 np.testing.assert_array_equal(buf.to_host(np.uint32, g.V), ref.view(np.uint32), err_msg=str(cfg))
             finally:
                 buf.free()
+
+
+def test_pagerank_autotune_workgroup_size():
+    """The tuner's PageRank mode picks the gather's workgroup size (256, 512
+    or 1024 threads, the LDS hub stage growing with it); the ranks keep their
+    bits, and set_config refuses other sizes."""
+    csr = pg.gen_rmat(15, 16, seed=4)
+    with pg.Graph.from_csr(csr, weighted=False) as g:
+        want, _ = g.pagerank(0.85, 20)
+        say("case pagerank autotune")
+        c = g.autotune(np.array([0], np.uint32), what=pg.TUNE_PAGERANK)
+        assert c.pr_block in (256, 512, 1024) and g.get_config().pr_block == c.pr_block
+        np.testing.assert_array_equal(g.pagerank(0.85, 20)[0].view(np.uint32), want.view(np.uint32))
+        for bad in (128, 300, 2048):
+            with pytest.raises(pg.PtgError) as e:
+                g.tune(pr_block=bad)
+            assert e.value.code == pg.ERR_INVALID_ARG
 
 
 def random_directed(rng, V, E, wmax):

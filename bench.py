@@ -40,7 +40,8 @@ After the timed steps (none of this is in `value`):
            BFS / 30% SSSP / 15% CC / 5% PageRank(10), Poisson arrivals at
            700/s (seed 4, near the plain path's saturation) under a 24 GiB
            query-state budget (the graphs' batched state alone would take
-           about 27 GB, so the budget forces evictions): the plain per-query path
+           about 27 GB, so the budget forces evictions), each graph autotuned
+           first for single queries and batches: the plain per-query path
            (one at a time) vs the server (continuous batching); p50/p99,
            mean batch and evictions at that load, and queries/s with every
            request queued at once; checksums must match.
@@ -193,15 +194,25 @@ def serving_leg(args, pg, np, cores, log):
     SSSP / 15% CC / 5% PageRank (10 iterations), uniform graphs and sources;
     the plain per-query path (one query at a time, no batching) against the
     server (continuous batching, coalescing, one worker per graph) at the same
-    offered load, then both saturated with every request at once."""
+    offered load, then both saturated with every request at once. Each graph
+    is autotuned first (single-query and batch knobs), and both servers use
+    the tuned graphs."""
     scales = [int(x) for x in args.serve_scales.split(",") if x]
     graphs = []
     t = time.time()
+    tune_rng = np.random.default_rng(5)
     for sc in scales:
         c = pg.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=1, threads=cores).add_weights(1, 64, seed=2, threads=cores)
-        graphs.append(pg.Graph.from_csr(c))
+        gr = pg.Graph.from_csr(c)
+        if not args.no_tune:
+            # the autotuner picks each graph's knobs for both paths (single
+            # queries and batches) on 64 of its own sources
+            deg = np.diff(c.offsets.astype(np.int64))
+            ts = np.array(tune_rng.choice(np.nonzero(deg > 0)[0], 64, replace=False), dtype=np.uint32)
+            gr.autotune(ts, what=pg.TUNE_BFS | pg.TUNE_SSSP | pg.TUNE_BFS_BATCH | pg.TUNE_SSSP_BATCH)
+        graphs.append(gr)
         del c
-    log("serving: %d graphs (scales %s) ready in %.1fs" % (len(graphs), scales, time.time() - t))
+    log("serving: %d graphs (scales %s) ready and tuned in %.1fs" % (len(graphs), scales, time.time() - t))
     rng = np.random.default_rng(4)
     N = args.serve_requests
     kinds = rng.choice([pg.Q_BFS, pg.Q_SSSP, pg.Q_CC, pg.Q_PAGERANK], N, p=[0.5, 0.3, 0.15, 0.05])

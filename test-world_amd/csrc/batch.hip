@@ -92,6 +92,8 @@ struct BatchEnv {
     const uint2 *cwin;         // SSSP pull rounds: in-edge (source, weight) pairs; null = push only
     const uint8_t *pmode;      // per round (< kBRing): pull when the previous batch's round was dense
     unsigned long long pull_prev;  // and the previous round of this batch expanded >= this many edges
+    unsigned long long pull_dense; // E x batch_pull: a dense round (the rule of a graph's first batch)
+    uint32_t learned;          // pmode holds a previous batch's rounds
     uint32_t sentinel;         // SSSP: unreached value
     BCtr *ring;
     BParams *p;
@@ -493,9 +495,17 @@ __device__ __forceinline__ void relax_chunk(const BatchEnv &e, const ChunkHead &
 // Whether round R pulls (the push kernel then leaves the expansion to
 // k_mssp_pull): an expand round of a graph with in-pairs, learned dense in
 // the previous batch, whose previous round expanded enough edges (munf of
-// the input entry).
+// the input entry). A graph's first batch has nothing learned: its round R
+// pulls while round R - 1 was dense and expanded no fewer edges than round
+// R - 2 (entry R - 1), so the tail after the dense rounds pushes again.
 __device__ __forceinline__ bool pull_round(const BatchEnv &e, uint32_t R, const BSum &s) {
-    return e.cwin && s.nf > 0 && R < kBRing && e.pmode[R] && s.munf >= e.pull_prev;
+    if (!e.cwin || s.nf == 0 || R >= kBRing) return false;
+    if (e.learned) return e.pmode[R] && s.munf >= e.pull_prev;
+    if (s.munf < e.pull_dense || R < 2) return false;
+    const BCtr *c = e.ring + (R - 1) % kBRing;
+    unsigned long long before = 0;
+    for (int i = 0; i < (int)kShards; i++) before += c->munf[i];
+    return s.munf >= before;
 }
 
 // The loads a pull chunk needs before its rows: the vertex's distance in
@@ -859,6 +869,7 @@ static int learn_pull_rounds(ptg_graph *g, uint32_t rounds) {
             for (uint32_t i = 0; i < kShards; i++) vis += b->h_ring[r + 1].munf[i];
             m[r] = (double)vis >= dense ? 1 : 0;
         }
+    g->batch_pull_learned = true;
     if (memcmp(m, g->batch_pull_rounds, kBRing) != 0) {
         memcpy(g->batch_pull_rounds, m, kBRing);
         return copy_sync(g, b->pmode, g->batch_pull_rounds, kBRing, hipMemcpyHostToDevice);
@@ -899,6 +910,8 @@ static BatchEnv make_batch_env(ptg_graph *g, int sym, bool sssp) {
         e.pmode = b->pmode;
         const float f = g->cfg.batch_pull > 0 ? g->cfg.batch_pull : kBatchPull;
         e.pull_prev = (unsigned long long)((double)g->E * f / 8.0);
+        e.pull_dense = (unsigned long long)((double)g->E * f);
+        e.learned = g->batch_pull_learned ? 1u : 0u;
     }
     return e;
 }

@@ -211,6 +211,7 @@ struct ptg_graph {
     // batched SSSP: the rounds of the next batch that pull, learned from the
     // last batch (kept when the batch state is released, e.g. by a server)
     uint8_t batch_pull_rounds[64] = {};
+    bool batch_pull_learned = false;
     // PTG_FLAG_STREAM: host-resident edges and the device chunk cache (stream.hip).
     ptg::StreamState *streamed = nullptr;
 

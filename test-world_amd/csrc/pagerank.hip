@@ -402,8 +402,9 @@ int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats
 // PageRank without allocation: the per-iteration buffers live in a fixed
 // device array of kPrScratchBytes, one copy per GPU, shared by every call on
 // that GPU (calls that overlap on one GPU race on it). Every word a run reads
-// is written earlier in the same run (k_pr_init, the D0 pair, then each
-// iteration's partials before the next reads them), so nothing is cleared.
+// is written earlier in the same run (k_pr_init's contributions, then each
+// iteration's partials and base value before the next reads them; iteration
+// 0 takes the dangling mass as a kernel argument), so nothing is cleared.
 // ---------------------------------------------------------------------------
 
 static __device__ __align__(256) uint8_t scratch[kPrScratchBytes];

@@ -232,7 +232,8 @@ PTG_API int ptg_sssp(ptg_graph *g, uint32_t src, uint32_t *dist_out, ptg_stats *
 PTG_API int ptg_cc(ptg_graph *g, uint32_t *labels_out, ptg_stats *st);
 /* prCaculateInShare (main.cu:891-959) has no arithmetic in the reference; this
  * defines it: r0 = 1/V, r' = (1-d)/V + d * (sum_{u->v} r(u)/outdeg(u) + D/V),
- * D = rank mass of vertices with outdeg 0; FP32 with compensated sums. */
+ * D = rank mass of vertices with outdeg 0. FP32 ranks from exact 64-bit
+ * fixed-point sums: the same bits for a resident or a streamed graph. */
 PTG_API int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ranks_out,
                  ptg_stats *st);
 /* ptg_pagerank that allocates nothing. No counterpart in the reference; it is

@@ -134,4 +134,31 @@ __device__ __forceinline__ void count_found(BinCounters *ctr, bool pred, uint32_
     }
 }
 
+
+// PageRank sums as unsigned 64-bit fixed point with 63 fraction bits
+// (Q1.63). Each FP32 term in [0, 1] is truncated to a multiple of 2^-63 and
+// added exactly, so a sum does not depend on the order or grouping of its
+// terms. The resident and streamed PageRank kernels both round through
+// pr_base_value and pr_rank, so they give the same bits.
+__device__ __forceinline__ unsigned long long fx63(float x) {
+    const uint32_t b = __float_as_uint(x);
+    const int e = (int)((b >> 23) & 0xFFu);
+    if (e == 0) return 0;  // zero and subnormals
+    const unsigned long long m = (b & 0x7FFFFFu) | 0x800000u;
+    const int sh = e - 87;  // x = m * 2^(e - 150), so x * 2^63 = m * 2^(e - 87); e <= 127
+    return sh >= 0 ? m << sh : (sh > -64 ? m >> -sh : 0ull);
+}
+
+__device__ __forceinline__ double fx63_to_f64(unsigned long long s) { return (double)s * 0x1p-63; }
+
+// base = (1-d)/V + d * D/V in FP64, rounded once to FP32.
+__device__ __forceinline__ float pr_base_value(float damping, float inv_v, double dang) {
+    return (float)((1.0 - (double)damping) * (double)inv_v + (double)damping * dang * (double)inv_v);
+}
+
+// r = base + d * sum in FP64 from the exact sum, rounded once to FP32.
+__device__ __forceinline__ float pr_rank(float base, float damping, unsigned long long sum) {
+    return (float)((double)base + (double)damping * fx63_to_f64(sum));
+}
+
 }  // namespace ptg

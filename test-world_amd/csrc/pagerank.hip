@@ -36,18 +36,6 @@ namespace ptg {
 // 20352 B, so 8 workgroups of 256 threads (or 4 of 512, 2 of 1024) fit a CU.
 constexpr uint32_t kPrHubLds = 5088;
 
-// x in [0, 1] as Q1.63, truncated (zero and subnormals give 0).
-__device__ __forceinline__ unsigned long long fx63(float x) {
-    const uint32_t b = __float_as_uint(x);
-    const int e = (int)((b >> 23) & 0xFFu);
-    if (e == 0) return 0;
-    const unsigned long long m = (b & 0x7FFFFFu) | 0x800000u;
-    const int sh = e - 87;  // x = m * 2^(e - 150), so x * 2^63 = m * 2^(e - 87); e <= 127
-    return sh >= 0 ? m << sh : (sh > -64 ? m >> -sh : 0ull);
-}
-
-__device__ __forceinline__ double fx63_to_f64(unsigned long long s) { return (double)s * 0x1p-63; }
-
 // Sum over the workgroup (Waves waves); every thread gets the result.
 template <uint32_t Waves>
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long x, unsigned long long *lds) {
@@ -88,7 +76,7 @@ struct PrVertex {
     // r = base + d * sum (rounded once); contrib = r / outdeg (0 for a
     // dangling vertex, whose r joins the dangling mass instead).
     __device__ __forceinline__ void finalize(uint32_t v, unsigned long long sum) const {
-        const float r = (float)((double)base + (double)a.damping * fx63_to_f64(sum));
+        const float r = pr_rank(base, a.damping, sum);
         const uint32_t od = a.ooff[v + 1] - a.ooff[v];
         a.cout[v] = od ? r / (float)od : 0.0f;
         if (a.rank_out) a.rank_out[v] = r;
@@ -108,7 +96,7 @@ __device__ __forceinline__ float pr_base(const PrArgs &a, unsigned long long *ld
         for (uint32_t j = threadIdx.x; j < a.combine_blocks; j += TB) t += a.cpart_in[j];
         dang = fx63_to_f64(block_sum<TB / 64>(t, lds));
     }
-    return (float)((1.0 - (double)a.damping) * (double)a.inv_v + (double)a.damping * dang * (double)a.inv_v);
+    return pr_base_value(a.damping, a.inv_v, dang);
 }
 
 // Gathers every vertex's in-neighbour contributions with TB threads per

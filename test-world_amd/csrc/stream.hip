@@ -74,9 +74,9 @@ struct StreamState {
     uint32_t *d_bits[2] = {};
     unsigned long long *d_cnt = nullptr, *h_cnt = nullptr;  // 2 x kShards
     float *d_contrib[2] = {};
-    float *d_ppart = nullptr;  // per piece (hi, lo)
-    float *d_dpart = nullptr;  // 2 iterations x chunks x kStreamGrid x (hi, lo)
-    float *d_base = nullptr;   // this iteration's base value + D0 (hi, lo)
+    unsigned long long *d_ppart = nullptr;  // per piece: Q1.63 sum
+    unsigned long long *d_dpart = nullptr;  // 2 iterations x chunks x kStreamGrid: Q1.63 dangling mass
+    float *d_base = nullptr;                // this iteration's base value
     uint64_t clock = 0;
     uint64_t loads = 0, hits = 0, bytes = 0;
 };
@@ -225,34 +225,6 @@ __global__ __launch_bounds__(kBlock) void k_stream_all(uint32_t V, uint32_t *bit
     }
 }
 
-// Compensated FP32 sums (TwoSum) in a fixed order.
-struct SCF {
-    float hi, lo;
-};
-
-__device__ __forceinline__ void scf_add(SCF &a, float x) {
-    const float s = a.hi + x;
-    const float bb = s - a.hi;
-    a.lo += (a.hi - (s - bb)) + (x - bb);
-    a.hi = s;
-}
-
-__device__ __forceinline__ SCF scf_merge(SCF a, SCF b) {
-    const float s = a.hi + b.hi;
-    const float bb = s - a.hi;
-    return SCF{s, (a.lo + b.lo) + ((a.hi - (s - bb)) + (b.hi - bb))};
-}
-
-__device__ __forceinline__ SCF wave_scf_sum(SCF a) {
-    for (int d = 32; d >= 1; d >>= 1) {
-        SCF b;
-        b.hi = __shfl_xor(a.hi, d, 64);
-        b.lo = __shfl_xor(a.lo, d, 64);
-        a = scf_merge(a, b);
-    }
-    return a;
-}
-
 struct PrStep {
     const uint32_t *off;
     const float *cin;
@@ -262,36 +234,36 @@ struct PrStep {
     float damping;
 };
 
-__device__ __forceinline__ void pr_finish(const PrStep &p, uint32_t v, float sum, float base, SCF &dang) {
-    const float r = base + p.damping * sum;
+// Sums are Q1.63 fixed point (fx63), added exactly, so chunking, piece
+// boundaries and slot order change no bit of the result.
+__device__ __forceinline__ void pr_finish(const PrStep &p, uint32_t v, unsigned long long sum, float base,
+                                          unsigned long long &dang) {
+    const float r = pr_rank(base, p.damping, sum);
     const uint32_t od = p.off[v + 1] - p.off[v];
     p.cout[v] = od ? r / (float)od : 0.0f;
     if (p.rank) p.rank[v] = r;
-    if (!od) scf_add(dang, r);
+    if (!od) dang += fx63(r);
 }
 
 // PageRank over one resident chunk: hub pieces leave partial sums, every
 // other vertex of the chunk is finished here; the workgroup's dangling mass
 // goes to its fixed slot.
 __global__ __launch_bounds__(kBlock) void k_stream_pr(ChunkArgs c, PrStep p, const uint32_t *__restrict__ pe0,
-                                                      const uint32_t *__restrict__ pe1, float *ppart,
-                                                      float *dpart) {
-    __shared__ SCF lds[kWavesPerBlock];
+                                                      const uint32_t *__restrict__ pe1, unsigned long long *ppart,
+                                                      unsigned long long *dpart) {
+    __shared__ unsigned long long lds[kWavesPerBlock];
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     const float base = p.base[0];
     for (uint32_t i = c.pb + gw; i < c.pe; i += nw) {
-        SCF a{0.0f, 0.0f};
+        unsigned long long a = 0;
         const uint64_t e1 = pe1[i];
-        for (uint64_t e = pe0[i] + lane; e < e1; e += 64) scf_add(a, p.cin[c.slot[e - c.eb]]);
-        a = wave_scf_sum(a);
-        if (lane == 0) {
-            ppart[2ull * i] = a.hi;
-            ppart[2ull * i + 1] = a.lo;
-        }
+        for (uint64_t e = pe0[i] + lane; e < e1; e += 64) a += fx63(p.cin[c.slot[e - c.eb]]);
+        a = wave_sum_u64(a);
+        if (lane == 0) ppart[i] = a;
     }
-    SCF dang{0.0f, 0.0f};
+    unsigned long long dang = 0;
     for (uint64_t base_v = (uint64_t)c.vh + (uint64_t)gw * 64; base_v < c.ve; base_v += (uint64_t)nw * 64) {
         const uint64_t v = base_v + lane;
         const bool inb = v < c.ve;
@@ -300,68 +272,66 @@ __global__ __launch_bounds__(kBlock) void k_stream_pr(ChunkArgs c, PrStep p, con
             b = p.off[v];
             d = p.off[v + 1] - b;
         }
-        SCF mine{0.0f, 0.0f};
+        unsigned long long mine = 0;
         uint64_t mm = ballot(inb && d > 32);
         while (mm) {
             const uint32_t q = lowest_lane(mm);
             mm &= mm - 1;
             const uint64_t bq = read_lane64(b, q), dq = read_lane64(d, q);
-            SCF a{0.0f, 0.0f};
-            for (uint64_t k = lane; k < dq; k += 64) scf_add(a, p.cin[c.slot[bq + k - c.eb]]);
-            a = wave_scf_sum(a);
+            unsigned long long a = 0;
+            for (uint64_t k = lane; k < dq; k += 64) a += fx63(p.cin[c.slot[bq + k - c.eb]]);
+            a = wave_sum_u64(a);
             if (lane == q) mine = a;
         }
         if (inb && d <= 32)
-            for (uint64_t k = 0; k < d; k++) scf_add(mine, p.cin[c.slot[b + k - c.eb]]);
-        if (inb) pr_finish(p, (uint32_t)v, mine.hi + mine.lo, base, dang);
+            for (uint64_t k = 0; k < d; k++) mine += fx63(p.cin[c.slot[b + k - c.eb]]);
+        if (inb) pr_finish(p, (uint32_t)v, mine, base, dang);
     }
-    dang = wave_scf_sum(dang);
+    dang = wave_sum_u64(dang);
     if (lane == 0) lds[threadIdx.x >> 6] = dang;
     __syncthreads();
     if (threadIdx.x == 0) {
-        SCF t = lds[0];
-        for (int w = 1; w < kWavesPerBlock; w++) t = scf_merge(t, lds[w]);
-        dpart[2ull * blockIdx.x] = t.hi;
-        dpart[2ull * blockIdx.x + 1] = t.lo;
+        unsigned long long t = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) t += lds[w];
+        dpart[blockIdx.x] = t;
     }
 }
 
 // Finishes the hub vertices of one chunk: one wave per run of pieces.
 __global__ __launch_bounds__(kBlock) void k_stream_pr_hubs(uint32_t rb, uint32_t re, const uint32_t *__restrict__ rv,
                                                            const uint32_t *__restrict__ rpb,
-                                                           const uint32_t *__restrict__ rpe, const float *ppart,
-                                                           PrStep p) {
+                                                           const uint32_t *__restrict__ rpe,
+                                                           const unsigned long long *ppart, PrStep p) {
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     const float base = p.base[0];
     for (uint32_t r = rb + gw; r < re; r += nw) {
-        SCF a{0.0f, 0.0f};
-        for (uint32_t i = rpb[r] + lane; i < rpe[r]; i += 64) a = scf_merge(a, SCF{ppart[2ull * i], ppart[2ull * i + 1]});
-        a = wave_scf_sum(a);
+        unsigned long long a = 0;
+        for (uint32_t i = rpb[r] + lane; i < rpe[r]; i += 64) a += ppart[i];
+        a = wave_sum_u64(a);
         if (lane == 0) {
-            SCF unused{0.0f, 0.0f};
-            pr_finish(p, rv[r], a.hi + a.lo, base, unused);  // hubs have out-edges: never dangling
+            unsigned long long unused = 0;
+            pr_finish(p, rv[r], a, base, unused);  // hubs have out-edges: never dangling
         }
     }
 }
 
-// base = (1-d)/V + d*D/V with D summed over every chunk's workgroup slots in
-// index order (or D0 for the first iteration). One workgroup.
-__global__ __launch_bounds__(kBlock) void k_stream_pr_base(const float *dpart, uint32_t nslots, const float *d0,
-                                                           float damping, float inv_v, float *base) {
-    __shared__ SCF lds[kWavesPerBlock];
-    SCF t{0.0f, 0.0f};
-    if (!d0)
-        for (uint32_t j = threadIdx.x; j < nslots; j += kBlock) t = scf_merge(t, SCF{dpart[2ull * j], dpart[2ull * j + 1]});
-    t = wave_scf_sum(t);
+// base = (1-d)/V + d*D/V with D summed over every chunk's workgroup slots,
+// or d0 (dangling vertices / V) for the first iteration. One workgroup.
+__global__ __launch_bounds__(kBlock) void k_stream_pr_base(const unsigned long long *dpart, uint32_t nslots, bool first,
+                                                           double d0, float damping, float inv_v, float *base) {
+    __shared__ unsigned long long lds[kWavesPerBlock];
+    unsigned long long t = 0;
+    if (!first)
+        for (uint32_t j = threadIdx.x; j < nslots; j += kBlock) t += dpart[j];
+    t = wave_sum_u64(t);
     if (lane_id() == 0) lds[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
-        SCF s = lds[0];
-        for (int w = 1; w < kWavesPerBlock; w++) s = scf_merge(s, lds[w]);
-        const float dang = d0 ? d0[0] + d0[1] : s.hi + s.lo;
-        base[0] = (1.0f - damping) * inv_v + damping * (dang * inv_v);
+        unsigned long long sum = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) sum += lds[w];
+        base[0] = pr_base_value(damping, inv_v, first ? d0 : fx63_to_f64(sum));
     }
 }
 
@@ -1001,17 +971,14 @@ int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, 
     } else {
         k_stream_pr_init<<<std::min<uint32_t>((V + kBlock - 1) / kBlock, g->grid), kBlock, 0, g->stream>>>(
             V, g->out.off, r0, s->d_contrib[0]);
-        // D0 = (vertices without out-edges) / V as (hi, lo)
-        const double d0v = (double)s->zeros / (double)V;
-        const float d0[2] = {(float)d0v, (float)(d0v - (double)(float)d0v)};
-        if ((rc = copy_sync(g, s->d_base + 2, d0, 8, hipMemcpyHostToDevice))) return rc;
+        const double d0 = (double)s->zeros / (double)V;  // D0 = (vertices without out-edges) / V
         std::vector<char> pending(C, 0);
         const uint64_t slots_per_iter = (uint64_t)C * kStreamGrid;
         for (int t = 0; t < iters; t++) {
-            float *dprev = s->d_dpart + 2 * slots_per_iter * ((t + 1) & 1);
-            float *dcur = s->d_dpart + 2 * slots_per_iter * (t & 1);
-            k_stream_pr_base<<<1, kBlock, 0, g->stream>>>(dprev, (uint32_t)slots_per_iter, t == 0 ? s->d_base + 2 : nullptr,
-                                                         damping, r0, s->d_base);
+            unsigned long long *dprev = s->d_dpart + slots_per_iter * ((t + 1) & 1);
+            unsigned long long *dcur = s->d_dpart + slots_per_iter * (t & 1);
+            k_stream_pr_base<<<1, kBlock, 0, g->stream>>>(dprev, (uint32_t)slots_per_iter, t == 0, d0, damping, r0,
+                                                         s->d_base);
             PrStep p{g->out.off, s->d_contrib[t & 1], s->d_contrib[(t + 1) & 1], t == iters - 1 ? ranks : nullptr,
                      s->d_base, damping};
             // alternate the order so the chunks left resident by one pass start the next
@@ -1022,7 +989,7 @@ int run_pagerank_streamed(ptg_graph *g, float damping, int iters, float *ranks, 
                 if ((rc = acquire_chunk(g, c, pending, false, &sl))) return rc;
                 pending[c] = 0;
                 k_stream_pr<<<kStreamGrid, kBlock, 0, g->stream>>>(chunk_args(s, c, sl), p, s->d_pe0, s->d_pe1,
-                                                                   s->d_ppart, dcur + 2ull * c * kStreamGrid);
+                                                                   s->d_ppart, dcur + (uint64_t)c * kStreamGrid);
                 const ChunkDesc &ch = s->chunks[c];
                 if (ch.re > ch.rb)
                     k_stream_pr_hubs<<<std::min<uint32_t>((ch.re - ch.rb + 3) / 4, kStreamGrid), kBlock, 0,

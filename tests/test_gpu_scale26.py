@@ -4,7 +4,8 @@ streamed in 256 MB chunks into a device cache under a 4 GB budget. BFS from
 vertex 0 is compared bit for bit with the oracle's reference loop and
 PageRank (10 iterations) within 1e-6 L1 of the FP64 oracle. The same graph
 resident in HBM (8.4 GB of columns, offsets near 2^31) is checked the same
-way, which covers index widths at this edge count."""
+way, which covers index widths at this edge count, and its PageRank must equal
+the streamed graph's bit for bit."""
 import os
 import sys
 
@@ -47,7 +48,7 @@ def oracle26(csr26):
     T = host_threads()
     ptr, col = csr26.node_pointers, csr26.cols
     lv = oracle.ref_loop(oracle.BFS, ptr, col, src=0, threads=T)[0]
-    return {"bfs": lv, "pr": None}
+    return {"bfs": lv, "pr": None, "pr_streamed": None}
 
 
 def pr_oracle(csr26, o):
@@ -81,6 +82,7 @@ def test_streamed_pagerank_at_scale26(csr26, oracle26, streamed26):
     l1 = float(np.abs(r.astype(np.float64) - pr_oracle(csr26, oracle26)).sum())
     say("rmat26 streamed pagerank L1 %.3e" % l1)
     assert l1 <= PR_L1
+    oracle26["pr_streamed"] = r
 
 
 def test_resident_at_scale26(csr26, oracle26):
@@ -93,5 +95,9 @@ def test_resident_at_scale26(csr26, oracle26):
         l1 = float(np.abs(r.astype(np.float64) - pr_oracle(csr26, oracle26)).sum())
         say("rmat26 resident pagerank L1 %.3e" % l1)
         assert l1 <= PR_L1
+        if oracle26.get("pr_streamed") is not None:
+            # exact sums: streamed and resident ranks are the same bits
+            np.testing.assert_array_equal(r.view(np.uint32), oracle26["pr_streamed"].view(np.uint32))
+            say("rmat26 streamed == resident pagerank bits")
         lab, _ = g.cc()
         np.testing.assert_array_equal(lab, oracle.cc_union_find(csr26.node_pointers, csr26.cols))

@@ -1,8 +1,8 @@
 """GPU parity of streamed graphs (PTG_FLAG_STREAM): edges (and weights) in
 pinned host memory, a device chunk cache under a budget. BFS levels, SSSP
 distances and CC labels must equal the oracle's bit for bit and PageRank must
-stay within 1e-6 L1 of the FP64 oracle, whatever the chunk size and however
-many chunks are evicted."""
+stay within 1e-6 L1 of the FP64 oracle and equal the resident graph's ranks bit
+for bit, whatever the chunk size and however many chunks are evicted."""
 import os
 import sys
 
@@ -73,6 +73,8 @@ def test_streamed_rmat_bfs_pagerank(case):
         csr = pg.gen_rmat(12, 16, seed=1)
         ptr, col = csr.node_pointers, csr.cols
     say("case streamed %s V=%d E=%d" % (case, ptr.size, col.size))
+    with pg.Graph(ptr, col) as rg:
+        resident_pr = {it: rg.pagerank(0.85, it)[0] for it in (0, 1, 10)}
     cfg = pg.default_config(flags=pg.FLAG_STREAM, chunk_bytes=chunk)
     if small:
         cfg.device_budget_bytes = state_budget(ptr, col, chunk, 0)
@@ -98,6 +100,9 @@ def test_streamed_rmat_bfs_pagerank(case):
             r, st = g.pagerank(0.85, iters)
             l1 = float(np.abs(r.astype(np.float64) - oracle.pagerank_f64(ptr, col, 0.85, iters)).sum())
             assert l1 <= PR_L1, (iters, l1)
+            # exact sums: the streamed ranks are the resident ranks, bit for bit
+            np.testing.assert_array_equal(r.view(np.uint32), resident_pr[iters].view(np.uint32),
+                                          err_msg="streamed vs resident pagerank, %d iterations" % iters)
         r2, st2 = g.pagerank(0.85, 10)
         assert np.array_equal(r.view(np.uint32), r2.view(np.uint32))  # deterministic
         assert loads > 0  # the first query copies its chunks in

@@ -121,7 +121,16 @@ typedef struct ptg_config {
                                      (0 = 256). The hub contributions it stages in LDS grow with
                                      it (5088, 10176, 20352 vertices), so a CU holds the same 160
                                      KB of LDS; the ranks are the same bits for every value */
+    uint32_t pr_layout;           /* PageRank on a resident graph: PTG_PR_AUTO (binned when the
+                                     (source, destination partition) pairs come to at most half
+                                     the edges, else the in-list pull), PTG_PR_PULL or
+                                     PTG_PR_BINNED. Every layout gives the same bits */
 } ptg_config;
+
+/* ptg_config.pr_layout */
+#define PTG_PR_AUTO 0u
+#define PTG_PR_PULL 1u
+#define PTG_PR_BINNED 2u
 
 /* Per-query statistics (the reference prints `iter` and `nodeSum`,
  * ptGraph/main.cu:1109-1113; these are their counterparts). */
@@ -212,7 +221,7 @@ PTG_API int ptg_graph_frontier_sizes(const ptg_graph *g, uint64_t *sizes, uint32
 PTG_API int ptg_graph_set_flags(ptg_graph *g, uint32_t flags);
 /* Replaces the graph's tuning: td_to_bu, bu_to_td, sssp_delta,
  * sssp_defer_degree, sssp_defer_window, batch_delta, batch_grid, batch_sssp_grid,
- * batch_td_to_bu, batch_chunk, batch_pull, pr_block, stream_pinned, PTG_FLAG_TIMING and, for a
+ * batch_td_to_bu, batch_chunk, batch_pull, pr_block, pr_layout, stream_pinned, PTG_FLAG_TIMING and, for a
  * streamed graph, chunk_bytes (the chunks are cut again and the slot cache
  * resized; PTG_ERR_BUDGET, with the previous size kept, when fewer than two
  * slots would fit). Every other field must equal the graph's

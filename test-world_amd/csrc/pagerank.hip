@@ -281,6 +281,7 @@ uint64_t pr_state_need(const ptg_graph *g) {
         need += 2 * al(4ull * g->V) + al(8 * pr_part_words(g)) + al(8 * (chunks + 1));
     }
     if (!g->have_in) need += al(4ull * (g->V + 1)) + al(4 * g->E) + al(8ull * g->V);  // upper bound: transpose
+    if (pr_bins_wanted(g)) need += pr_bins_need(g);
     return need;
 }
 
@@ -293,6 +294,7 @@ void release_pr_state(ptg_graph *g) {
     gfree(g, g->d_lpart);
     g->d_part = g->d_lpart = nullptr;
     g->lchunks_cap = 0;
+    release_pr_bins(g);
 }
 
 // Per-iteration buffers of one PageRank run: contributions (2 x V floats),
@@ -381,6 +383,10 @@ static int pagerank_core(ptg_graph *g, float damping, int iters, float *ranks, p
 
 int run_pagerank(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
     int rc;
+    if (pr_bins_wanted(g)) {
+        bool done = false;
+        if ((rc = run_pagerank_bins(g, damping, iters, ranks, st, &done)) || done) return rc;
+    }
     if ((rc = ensure_in_edges(g)) || (rc = ensure_pr_state(g))) return rc;
     const PrBufs buf{{g->d_contrib[0], g->d_contrib[1]}, g->d_part, g->d_lpart};
     return pagerank_core(g, damping, iters, ranks, st, buf, true);

@@ -1,0 +1,606 @@
+// PageRank over destination partitions ("binned" PageRank) for gfx950.
+//
+// The same definition and the same exact Q1.63 sums as the in-list pull of
+// pagerank.hip (r' = base + d * sum, base = (1-d)/V + d*D/V), so the two give
+// the same bits. What changes is how a sum is gathered. The pull reads
+// r(u)/outdeg(u) at a random address for every edge, and a random 4-byte
+// read costs the same address-unit time whether it hits L2 or not. Here:
+//
+//   partitions  destinations in ranges of kPW = 16384 vertices; a workgroup
+//               keeps the 16384 sums of one range in LDS (128 KB of Q1.63);
+//   pairs       one per (source u, partition p) with an edge u -> v, v in p.
+//               bins[j] holds r(u)/outdeg(u) of pair j, written once per
+//               iteration when u is finished. Pairs are ordered by
+//               partition, then source. In the default degree order most of
+//               a vertex's edges go to the hubs, which share partition 0, so
+//               there are far fewer pairs than edges (0.16 E at R-MAT scale 22);
+//   entries     one 16-bit word per edge, in pair order and, within a pair,
+//               by destination: the destination's offset in its partition
+//               (14 bits) and, in bit 15, whether the edge starts a new pair.
+//               0x4000 is a null entry (padding to a multiple of 4);
+//   segments    every kSeg entries of a partition: the index of the pair that
+//               holds the entry before the segment, so a wave can start there;
+//   chunks      one workgroup's share of k_prb_acc: a partition, or a run
+//               of segments of a large one (the hub partitions). Each chunk
+//               leaves its LDS sums as a row of kPW partial sums;
+//   units       one workgroup's share of k_prb_fin: 1024 vertices of a
+//               partition of at most kRowsDirect rows (a thread per vertex
+//               adds its rows), or 64 vertices of a partition cut into more
+//               (the 16 waves split the rows).
+//
+// Per iteration, k_prb_acc streams the entries (2 bytes per edge) and the
+// bins (4 bytes per pair, read in order) and adds each edge's term into LDS
+// with a 64-bit LDS atomic; k_prb_fin adds up every vertex's rows and
+// finishes it: its rank, its share of the dangling mass and the next
+// iteration's bins, written through pair_pos (for each source, the bin index
+// of each of its pairs). Workgroup 0 of k_prb_acc sums the previous
+// iteration's dangling partials into the base value k_prb_fin uses. Kernel
+// boundaries order everything.
+//
+// The layout is built once per graph on the host from the device out-lists
+// (the edges u -> v themselves, so a directed graph needs no transpose) and
+// kept in host memory, so releasing the device copy and uploading it again
+// is a plain copy.
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "device_util.h"
+
+namespace ptg {
+
+constexpr uint32_t kPWBits = 14;
+constexpr uint32_t kPW = 1u << kPWBits;   // destinations per partition
+constexpr uint32_t kSeg = 4096;           // entries per segment
+constexpr uint32_t kAccBlock = 1024;      // threads of k_prb_acc / k_prb_fin
+constexpr uint32_t kAccWaves = kAccBlock / 64;
+constexpr uint32_t kTile = 64;            // vertices of a k_prb_fin unit whose rows the waves split
+constexpr uint32_t kRowsDirect = 4;       // at most this many rows: a thread per vertex adds them
+constexpr uint16_t kEntFirst = 0x8000, kEntNull = 0x4000, kEntDest = kPW - 1;
+
+struct PrbChunk {
+    uint64_t e0, e1;  // entries [e0, e1), within one partition
+    uint32_t seg0;    // segment of e0
+    uint32_t row;     // row of partial sums it writes
+};
+
+struct PrbUnit {
+    uint32_t v0;     // first vertex (within one partition)
+    uint32_t nv;     // vertices: up to kAccBlock, or kTile when nrows > kRowsDirect
+    uint32_t row0;   // the partition's rows
+    uint32_t nrows;
+};
+
+// Host copy of the layout.
+struct PrBinsHost {
+    std::vector<uint16_t> ent;
+    std::vector<uint32_t> pos, pb, seg_cur;
+    std::vector<PrbChunk> chunks;
+    std::vector<PrbUnit> units;
+    uint32_t n_rows = 0;
+    uint64_t n_pairs = 0;
+};
+
+struct PrBins {
+    PrBinsHost h;
+    bool built = false;
+    bool usable = false;  // auto layout: false when pairs would exceed half the edges
+    // device copy
+    uint16_t *ent = nullptr;
+    uint32_t *pos = nullptr, *pb = nullptr, *seg_cur = nullptr;
+    PrbChunk *chunks = nullptr;
+    PrbUnit *units = nullptr;
+    float *bins[2] = {};
+    unsigned long long *rows = nullptr;
+    unsigned long long *part = nullptr;  // 2 iterations x units, then the base value
+    bool on_device = false;
+};
+
+struct PrbArgs {
+    const uint16_t *ent;
+    const uint32_t *seg_cur;
+    const PrbChunk *chunks;
+    const PrbUnit *units;
+    const uint32_t *ooff;  // out offsets (outdeg)
+    const uint32_t *pb;    // V + 1: first pair of each source
+    const uint32_t *pos;   // bin index of every pair, by source
+    const float *bin_in;
+    float *bin_out;
+    float *rank_out;       // last iteration only
+    unsigned long long *rows;
+    const unsigned long long *part_in;  // previous iteration's dangling partials (one per unit)
+    unsigned long long *part_out;
+    float *base;           // this iteration's base value
+    uint32_t n_units;
+    uint32_t n_vert;
+    uint32_t last_pair;    // pairs - 1
+    bool first;            // iteration 0: the dangling mass is d0
+    bool init;             // k_prb_fin writes the bins of r0 = 1/V only
+    double d0;
+    float damping, inv_v;
+};
+
+// Workgroup 0 of k_prb_acc: base = (1-d)/V + d * D/V from the previous
+// iteration's dangling partials.
+__device__ __forceinline__ void prb_base(const PrbArgs &a, unsigned long long *lds) {
+    double dang = a.d0;
+    if (!a.first) {
+        unsigned long long t = 0;
+        for (uint32_t j = threadIdx.x; j < a.n_units; j += kAccBlock) t += a.part_in[j];
+        t = wave_sum_u64(t);
+        if (lane_id() == 0) lds[threadIdx.x >> 6] = t;
+        __syncthreads();
+        unsigned long long s = 0;
+        for (uint32_t w = 0; w < kAccWaves; w++) s += lds[w];
+        dang = fx63_to_f64(s);
+    }
+    if (threadIdx.x == 0) *a.base = pr_base_value(a.damping, a.inv_v, dang);
+}
+
+// Writes the contributions s_c[i] of the nv vertices whose first pairs are
+// s_pb[0..nv] to the bins of all their pairs; the workgroup's threads walk
+// the pairs in order (pair_pos reads coalesced), four at a time.
+__device__ __forceinline__ void prb_scatter(const PrbArgs &a, uint32_t nv, const uint32_t *s_pb, const float *s_c) {
+    const uint32_t k0 = s_pb[0], k1 = s_pb[nv];
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += 4 * kAccBlock) {
+        uint32_t at[4];
+        float c[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t kk = k + i * kAccBlock;
+            if (kk < k1) {
+                uint32_t lo = 0, hi = nv;  // s_pb[lo] <= kk < s_pb[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pb[mid] <= kk)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                at[i] = a.pos[kk];
+                c[i] = s_c[lo];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (k + i * kAccBlock < k1) a.bin_out[at[i]] = c[i];
+    }
+}
+
+// One workgroup per chunk: adds every entry's term into LDS sums of its
+// partition and writes them as the chunk's row.
+__global__ __launch_bounds__(kAccBlock) void k_prb_acc(PrbArgs a) {
+    __shared__ unsigned long long s_sum[kPW];
+    __shared__ unsigned long long s_red[kAccWaves];
+    const PrbChunk ch = a.chunks[blockIdx.x];
+    if (blockIdx.x == 0) prb_base(a, s_red);
+    for (uint32_t i = threadIdx.x; i < kPW; i += kAccBlock) s_sum[i] = 0;
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    uint32_t j = ch.seg0 + wave;
+    for (uint64_t s = ch.e0 + (uint64_t)wave * kSeg; s < ch.e1; s += (uint64_t)kAccWaves * kSeg, j += kAccWaves) {
+        const uint64_t se = s + kSeg < ch.e1 ? s + kSeg : ch.e1;
+        uint32_t cur = a.seg_cur[j];  // pair holding the entry before s
+        // four entries per lane; lanes past the end read nulls (no pair start)
+        auto load4 = [&](uint64_t g) -> uint64_t {
+            const uint64_t e = g + 4ull * lane;
+            return e < se ? *reinterpret_cast<const uint64_t *>(a.ent + e) : 0x4000400040004000ull;
+        };
+        uint64_t w4 = load4(s);
+        for (uint64_t g = s; g < se; g += 256) {
+            const uint64_t nxt = g + 256 < se ? load4(g + 256) : 0x4000400040004000ull;
+            const uint32_t cnt = (uint32_t)((w4 >> 15) & 1) + (uint32_t)((w4 >> 31) & 1) +
+                                 (uint32_t)((w4 >> 47) & 1) + (uint32_t)((w4 >> 63) & 1);
+            const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2), b2 = ballot(cnt & 4);
+            uint32_t pr = cur + rank_below(b0) + 2 * rank_below(b1) + 4 * rank_below(b2);
+            cur += (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+            // every lane's pair index is a valid bin (a null entry, or a lane
+            // past the end, repeats the pair before it); clamped regardless
+            float x[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                pr += (uint32_t)(w4 >> (16 * i + 15)) & 1u;
+                x[i] = a.bin_in[min(pr, a.last_pair)];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t d = (uint32_t)(w4 >> (16 * i)) & 0xFFFFu;
+                if (!(d & kEntNull))
+                    __hip_atomic_fetch_add(&s_sum[d & kEntDest], fx63(x[i]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            w4 = nxt;
+        }
+    }
+    __syncthreads();
+    unsigned long long *row = a.rows + (uint64_t)ch.row * kPW;
+    for (uint32_t i = threadIdx.x; i < kPW; i += kAccBlock) row[i] = s_sum[i];
+}
+
+// One workgroup per unit: adds up each vertex's rows, finishes it (rank,
+// dangling share, contribution) and writes its bins. With `init`, writes the
+// bins of r0 = 1/V instead.
+__global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
+    __shared__ unsigned long long s_part[kAccWaves][kTile];
+    __shared__ uint32_t s_pb[kAccBlock + 1];
+    __shared__ float s_c[kAccBlock];
+    __shared__ unsigned long long s_red[kAccWaves];
+    const PrbUnit u = a.units[blockIdx.x];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t off0 = u.v0 & kEntDest;
+    unsigned long long sum = 0;
+    if (!a.init) {
+        if (u.nrows > kRowsDirect) {
+            for (uint32_t r = wave; r < u.nrows; r += kAccWaves) sum += a.rows[(uint64_t)(u.row0 + r) * kPW + off0 + lane];
+            s_part[wave][lane] = sum;
+            __syncthreads();
+            sum = 0;
+            if (threadIdx.x < kTile)
+                for (uint32_t w = 0; w < kAccWaves; w++) sum += s_part[w][threadIdx.x];
+        } else if (threadIdx.x < u.nv) {
+            for (uint32_t r = 0; r < u.nrows; r++) sum += a.rows[(uint64_t)(u.row0 + r) * kPW + off0 + threadIdx.x];
+        }
+    }
+    unsigned long long dang = 0;
+    if (threadIdx.x < u.nv) {
+        const uint32_t v = u.v0 + threadIdx.x;
+        const uint32_t od = a.ooff[v + 1] - a.ooff[v];
+        float c;
+        if (a.init) {
+            c = od ? a.inv_v / (float)od : 0.0f;
+        } else {
+            const float r = pr_rank(*a.base, a.damping, sum);
+            if (a.rank_out) a.rank_out[v] = r;
+            if (!od) dang = fx63(r);
+            c = od ? r / (float)od : 0.0f;
+        }
+        s_c[threadIdx.x] = c;
+        s_pb[threadIdx.x] = a.pb[v];
+    }
+    if (threadIdx.x == 0) s_pb[u.nv] = a.pb[u.v0 + u.nv];
+    __syncthreads();
+    prb_scatter(a, u.nv, s_pb, s_c);
+    if (a.init) return;
+    dang = wave_sum_u64(dang);
+    if (lane == 0) s_red[wave] = dang;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < kAccWaves; w++) t += s_red[w];
+        a.part_out[blockIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host build of the layout.
+// ---------------------------------------------------------------------------
+
+static int build_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(h ? h : 1u, 16u));
+}
+
+template <class F>
+static void run_ranges(const std::vector<uint32_t> &vb, F f) {
+    std::vector<std::thread> th;
+    for (size_t t = 0; t + 1 < vb.size(); t++) th.emplace_back([=] { f((int)t, vb[t], vb[t + 1]); });
+    for (auto &x : th) x.join();
+}
+
+// Builds the layout from CSR out-lists (each list ascending). Returns false,
+// with nothing built, when `require_gain` and the pairs exceed half the edges.
+static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint32_t *col, uint32_t num_cus,
+                         bool require_gain, PrBinsHost *out) {
+    const uint32_t P = (V + kPW - 1) >> kPWBits;
+    // thread ranges of about equal edges
+    const int T = (int)std::max<uint64_t>(1, std::min<uint64_t>(build_threads(), (uint64_t)V / 4096 + 1));
+    std::vector<uint32_t> vb(T + 1, V);
+    vb[0] = 0;
+    for (int t = 1; t < T; t++) {
+        const uint64_t target = E * (uint64_t)t / (uint64_t)T;
+        vb[t] = (uint32_t)(std::upper_bound(off, off + V, (uint32_t)target) - off);
+        vb[t] = std::max(vb[t], vb[t - 1]);
+    }
+    // pass 1: pairs and entries per (thread, partition), pairs per source
+    std::vector<std::vector<uint64_t>> cp(T, std::vector<uint64_t>(P, 0)), ce(T, std::vector<uint64_t>(P, 0));
+    std::vector<uint32_t> np(V);
+    run_ranges(vb, [&](int t, uint32_t b, uint32_t e) {
+        auto &mp = cp[t];
+        auto &me = ce[t];
+        for (uint32_t u = b; u < e; u++) {
+            uint32_t n = 0, prev = 0xFFFFFFFFu;
+            for (uint32_t k = off[u]; k < off[u + 1]; k++) {
+                const uint32_t p = col[k] >> kPWBits;
+                me[p]++;
+                if (p != prev) {
+                    mp[p]++;
+                    n++;
+                    prev = p;
+                }
+            }
+            np[u] = n;
+        }
+    });
+    uint64_t pairs = 0;
+    for (int t = 0; t < T; t++)
+        for (uint32_t p = 0; p < P; p++) pairs += cp[t][p];
+    if (require_gain && pairs * 2 > E) return false;
+    // partition offsets (entries padded to a multiple of 4), thread cursors
+    std::vector<uint64_t> ent_off(P + 1, 0), pair_off(P + 1, 0);
+    std::vector<uint32_t> seg_off(P + 1, 0);
+    for (uint32_t p = 0; p < P; p++) {
+        uint64_t ne = 0, npp = 0;
+        for (int t = 0; t < T; t++) {
+            ne += ce[t][p];
+            npp += cp[t][p];
+        }
+        ent_off[p + 1] = ent_off[p] + ((ne + 3) & ~3ull);
+        pair_off[p + 1] = pair_off[p] + npp;
+        seg_off[p + 1] = seg_off[p] + (uint32_t)((ent_off[p + 1] - ent_off[p] + kSeg - 1) / kSeg);
+    }
+    std::vector<std::vector<uint64_t>> ec(T, std::vector<uint64_t>(P)), pc(T, std::vector<uint64_t>(P));
+    for (uint32_t p = 0; p < P; p++) {
+        uint64_t e = ent_off[p], q = pair_off[p];
+        for (int t = 0; t < T; t++) {
+            ec[t][p] = e;
+            pc[t][p] = q;
+            e += ce[t][p];
+            q += cp[t][p];
+        }
+    }
+    out->pb.assign((size_t)V + 1, 0);
+    for (uint32_t u = 0; u < V; u++) out->pb[u + 1] = out->pb[u] + np[u];
+    out->n_pairs = pairs;
+    out->pos.assign(pairs ? pairs : 1, 0);
+    out->ent.assign(ent_off[P] ? ent_off[P] : 4, kEntNull);
+    out->seg_cur.assign(seg_off[P] ? seg_off[P] : 1, 0);
+    // pass 2: entries, pair positions, segment starts
+    run_ranges(vb, [&](int t, uint32_t b, uint32_t e) {
+        auto &ecur = ec[t];
+        auto &pcur = pc[t];
+        for (uint32_t u = b; u < e; u++) {
+            uint32_t prev = 0xFFFFFFFFu, kp = out->pb[u];
+            uint32_t pp = 0;
+            for (uint32_t k = off[u]; k < off[u + 1]; k++) {
+                const uint32_t v = col[k];
+                const uint32_t p = v >> kPWBits;
+                const bool first = p != prev;
+                if (first) {
+                    pp = (uint32_t)pcur[p]++;
+                    out->pos[kp++] = pp;
+                    prev = p;
+                }
+                const uint64_t x = ecur[p]++;
+                out->ent[x] = (uint16_t)((v & kEntDest) | (first ? kEntFirst : 0));
+                const uint64_t rel = x - ent_off[p];
+                if (rel % kSeg == 0) out->seg_cur[seg_off[p] + rel / kSeg] = pp - (first ? 1u : 0u);
+            }
+        }
+    });
+    // chunks: a partition of at most 2 x target entries is one chunk; larger
+    // ones are cut into runs of whole segments of about target entries. Each
+    // chunk has a row; the units of a partition add up its rows.
+    const uint64_t target = std::max<uint64_t>(16ull * kSeg, ent_off[P] / std::max<uint32_t>(1, num_cus));
+    out->chunks.clear();
+    out->units.clear();
+    out->n_rows = 0;
+    for (uint32_t p = 0; p < P; p++) {
+        const uint64_t n = ent_off[p + 1] - ent_off[p];
+        const uint32_t row0 = out->n_rows;
+        if (n <= 2 * target) {
+            out->chunks.push_back(PrbChunk{ent_off[p], ent_off[p + 1], seg_off[p], out->n_rows++});
+        } else {
+            const uint64_t nseg = seg_off[p + 1] - seg_off[p];
+            const uint64_t k = (n + target - 1) / target;
+            const uint64_t per = (nseg + k - 1) / k;  // segments per chunk
+            for (uint64_t s = 0; s < nseg; s += per) {
+                const uint64_t e0 = ent_off[p] + s * kSeg;
+                const uint64_t e1 = std::min<uint64_t>(ent_off[p + 1], e0 + per * kSeg);
+                out->chunks.push_back(PrbChunk{e0, e1, seg_off[p] + (uint32_t)s, out->n_rows++});
+            }
+        }
+        const uint32_t nrows = out->n_rows - row0;
+        const uint32_t step = nrows > kRowsDirect ? kTile : kAccBlock;
+        const uint32_t v0 = p << kPWBits, v1 = std::min<uint32_t>(V, v0 + kPW);
+        for (uint32_t v = v0; v < v1; v += step) out->units.push_back(PrbUnit{v, std::min(step, v1 - v), row0, nrows});
+    }
+    // largest chunks first, so the tail of the launch is short ones
+    std::stable_sort(out->chunks.begin(), out->chunks.end(),
+                     [](const PrbChunk &x, const PrbChunk &y) { return x.e1 - x.e0 > y.e1 - y.e0; });
+    return true;
+}
+
+static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
+
+static uint64_t device_bytes_of(const PrBinsHost &h, uint32_t V) {
+    return al256(2 * h.ent.size()) + al256(4 * h.pos.size()) + al256(4ull * ((uint64_t)V + 1)) +
+           al256(4 * h.seg_cur.size()) + al256(sizeof(PrbChunk) * h.chunks.size()) +
+           al256(sizeof(PrbUnit) * h.units.size()) + 2 * al256(4 * h.pos.size()) +
+           al256(8ull * kPW * h.n_rows) + al256(8 * (2 * h.units.size() + 1));
+}
+
+static void free_device(ptg_graph *g, PrBins *b) {
+    gfree(g, b->ent);
+    gfree(g, b->pos);
+    gfree(g, b->pb);
+    gfree(g, b->seg_cur);
+    gfree(g, b->chunks);
+    gfree(g, b->units);
+    gfree(g, b->bins[0]);
+    gfree(g, b->bins[1]);
+    gfree(g, b->rows);
+    gfree(g, b->part);
+    b->ent = nullptr;
+    b->pos = b->pb = b->seg_cur = nullptr;
+    b->chunks = nullptr;
+    b->units = nullptr;
+    b->bins[0] = b->bins[1] = nullptr;
+    b->rows = b->part = nullptr;
+    b->on_device = false;
+}
+
+static int upload(ptg_graph *g, PrBins *b) {
+    const PrBinsHost &h = b->h;
+    int rc;
+    if ((rc = galloc(g, (void **)&b->ent, 2 * h.ent.size())) || (rc = galloc(g, (void **)&b->pos, 4 * h.pos.size())) ||
+        (rc = galloc(g, (void **)&b->pb, 4 * h.pb.size())) ||
+        (rc = galloc(g, (void **)&b->seg_cur, 4 * h.seg_cur.size())) ||
+        (rc = galloc(g, (void **)&b->chunks, sizeof(PrbChunk) * h.chunks.size())) ||
+        (rc = galloc(g, (void **)&b->units, sizeof(PrbUnit) * h.units.size())) ||
+        (rc = galloc(g, (void **)&b->bins[0], 4 * h.pos.size())) ||
+        (rc = galloc(g, (void **)&b->bins[1], 4 * h.pos.size())) ||
+        (rc = galloc(g, (void **)&b->rows, 8ull * kPW * h.n_rows)) ||
+        (rc = galloc(g, (void **)&b->part, 8 * (2 * h.units.size() + 1)))) {
+        free_device(g, b);
+        return rc;
+    }
+    if ((rc = copy_sync(g, b->ent, h.ent.data(), 2 * h.ent.size(), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, b->pos, h.pos.data(), 4 * h.pos.size(), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, b->pb, h.pb.data(), 4 * h.pb.size(), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, b->seg_cur, h.seg_cur.data(), 4 * h.seg_cur.size(), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, b->chunks, h.chunks.data(), sizeof(PrbChunk) * h.chunks.size(), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, b->units, h.units.data(), sizeof(PrbUnit) * h.units.size(), hipMemcpyHostToDevice))) {
+        free_device(g, b);
+        return rc;
+    }
+    b->on_device = true;
+    return PTG_OK;
+}
+
+// Builds the layout from the device out-lists.
+static int build_from_device(ptg_graph *g, bool require_gain, bool *usable) {
+    const uint32_t V = g->V;
+    const uint64_t E = g->E;
+    std::vector<uint32_t> off((size_t)V + 1), col(E ? E : 1);
+    int rc;
+    if ((rc = copy_sync(g, off.data(), g->out.off, 4ull * ((uint64_t)V + 1), hipMemcpyDeviceToHost)) ||
+        (E && (rc = copy_sync(g, col.data(), g->out.col, 4 * E, hipMemcpyDeviceToHost))))
+        return rc;
+    *usable = build_layout(V, E, off.data(), col.data(), (uint32_t)std::max(1, g->num_cus), require_gain, &g->prb->h);
+    return PTG_OK;
+}
+
+// Builds (once) and uploads the layout; *use = false when the graph takes
+// the in-list pull instead (automatic layout and more pairs than E / 2).
+static int ensure_bins(ptg_graph *g, bool *use) {
+    *use = false;
+    if (!g->prb) g->prb = new PrBins();
+    PrBins *b = g->prb;
+    const bool forced = g->cfg.pr_layout == PTG_PR_BINNED;
+    int rc;
+    if (!b->built) {
+        if ((rc = build_from_device(g, !forced, &b->usable))) return rc;
+        b->built = true;
+    }
+    if (!b->usable) {
+        if (!forced) return PTG_OK;
+        if ((rc = build_from_device(g, false, &b->usable))) return rc;  // refused earlier by the automatic layout
+    }
+    if (!b->on_device && (rc = upload(g, b))) return rc;
+    *use = true;
+    return PTG_OK;
+}
+
+bool pr_bins_wanted(const ptg_graph *g) { return g->cfg.pr_layout != PTG_PR_PULL; }
+
+uint64_t pr_bins_need(const ptg_graph *g) {
+    const PrBins *b = g->prb;
+    if (b && b->built) return b->usable && !b->on_device ? device_bytes_of(b->h, g->V) : 0;
+    // not built yet: entries, and pairs at most E / 2 (12 bytes each)
+    return al256(2 * (g->E + 4ull * ((g->V + kPW - 1) / kPW))) + 3 * al256(4 * (g->E / 2 + 1)) +
+           al256(4ull * ((uint64_t)g->V + 1)) + al256(8ull * kPW * 64) + (1u << 20);
+}
+
+void release_pr_bins(ptg_graph *g) {
+    if (g->prb) free_device(g, g->prb);
+}
+
+void destroy_pr_bins(ptg_graph *g) {
+    if (!g->prb) return;
+    free_device(g, g->prb);
+    delete g->prb;
+    g->prb = nullptr;
+}
+
+int run_pagerank_bins(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st, bool *done) {
+    *done = false;
+    bool use = false;
+    int rc = ensure_bins(g, &use);
+    if (rc || !use) return rc;
+    *done = true;
+    PrBins *b = g->prb;
+    const uint32_t V = g->V;
+    const float r0 = 1.0f / (float)V;
+    LaunchTimer timer(g);
+    PTG_HIP(hipEventRecord(g->ev0, g->stream));
+    if (iters == 0) {
+        union {
+            float f;
+            uint32_t u;
+        } cv;
+        cv.f = r0;
+        if ((rc = fill_u32(g, reinterpret_cast<uint32_t *>(ranks), V, cv.u))) return rc;
+    } else {
+        const uint32_t nc = (uint32_t)b->h.chunks.size(), nu = (uint32_t)b->h.units.size();
+        PrbArgs a;
+        memset(&a, 0, sizeof a);
+        a.ent = b->ent;
+        a.seg_cur = b->seg_cur;
+        a.chunks = b->chunks;
+        a.units = b->units;
+        a.ooff = g->out.off;
+        a.pb = b->pb;
+        a.pos = b->pos;
+        a.rows = b->rows;
+        a.base = reinterpret_cast<float *>(b->part + 2ull * nu);
+        a.n_units = nu;
+        a.n_vert = V;
+        a.last_pair = (uint32_t)(b->h.n_pairs ? b->h.n_pairs - 1 : 0);
+        a.d0 = (double)g->out.bins.n_zero / (double)V;  // D0 = (vertices without out-edges) / V
+        a.damping = damping;
+        a.inv_v = r0;
+        a.init = true;
+        a.bin_out = b->bins[0];
+        k_prb_fin<<<nu, kAccBlock, 0, g->stream>>>(a);
+        PTG_HIP(hipGetLastError());
+        a.init = false;
+        for (int t = 0; t < iters; t++) {
+            a.bin_in = b->bins[t & 1];
+            a.bin_out = b->bins[(t + 1) & 1];
+            a.rank_out = t == iters - 1 ? ranks : nullptr;
+            a.part_in = b->part + (uint64_t)nu * ((t + 1) & 1);
+            a.part_out = b->part + (uint64_t)nu * (t & 1);
+            a.first = t == 0;
+            timer.launch(k_prb_acc, dim3(nc), dim3(kAccBlock), a);
+            timer.launch_rider(k_prb_fin, dim3(nu), dim3(kAccBlock), a);
+            PTG_HIP(hipGetLastError());
+        }
+    }
+    PTG_HIP(hipEventRecord(g->ev1, g->stream));
+    PTG_HIP(hipEventSynchronize(g->ev1));
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        PTG_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        st->kernel_ns = (uint64_t)((double)ms * 1e6);
+        st->iterations = (uint32_t)iters;
+        st->reached = V;
+        st->edges_traversed = g->E * (uint64_t)iters;
+        if ((rc = timer.collect(st))) return rc;
+    }
+    return PTG_OK;
+}
+
+int pr_bins_info(const ptg_graph *g, uint64_t *pairs, uint32_t *chunks, uint32_t *rows) {
+    const PrBins *b = g->prb;
+    const bool ok = b && b->built && b->usable;
+    if (pairs) *pairs = ok ? b->h.n_pairs : 0;
+    if (chunks) *chunks = ok ? (uint32_t)b->h.chunks.size() : 0;
+    if (rows) *rows = ok ? b->h.n_rows : 0;
+    return PTG_OK;
+}
+
+}  // namespace ptg

@@ -121,6 +121,7 @@ struct Csr {
 namespace ptg {
 struct BatchState;   // batch.hip
 struct StreamState;  // stream.hip
+struct PrBins;       // pagerank_bins.hip
 }
 
 // The graph handle.
@@ -171,6 +172,7 @@ struct ptg_graph {
     unsigned long long *d_part = nullptr;   // PageRank dangling partials and the base value
     unsigned long long *d_lpart = nullptr;  // hub chunk sums (fixed point)
     uint32_t lchunks_cap = 0;
+    ptg::PrBins *prb = nullptr;  // binned PageRank layout (pagerank_bins.hip), built on first use
 
     // Per-launch timing of the main kernel (PTG_FLAG_TIMING).
     std::vector<hipEvent_t> tev;  // pairs: [2i] before, [2i+1] after launch i
@@ -264,6 +266,14 @@ int run_bfs(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_sssp(ptg_graph *g, uint32_t src, uint32_t *out, ptg_stats *st);
 int run_cc(ptg_graph *g, uint32_t *out, ptg_stats *st);
 int run_pagerank(ptg_graph *g, float damping, int iters, float *out, ptg_stats *st);
+// Binned PageRank (pagerank_bins.hip): *done = false when the graph takes the
+// in-list pull instead (pr_layout, or too many pairs for the automatic layout).
+int run_pagerank_bins(ptg_graph *g, float damping, int iters, float *out, ptg_stats *st, bool *done);
+bool pr_bins_wanted(const ptg_graph *g);
+uint64_t pr_bins_need(const ptg_graph *g);
+void release_pr_bins(ptg_graph *g);
+void destroy_pr_bins(ptg_graph *g);
+int pr_bins_info(const ptg_graph *g, uint64_t *pairs, uint32_t *chunks, uint32_t *rows);
 // PageRank with its per-iteration buffers in a fixed per-GPU device array of
 // kPrScratchBytes (pagerank.hip); allocates nothing.
 constexpr uint64_t kPrScratchBytes = 64ull << 20;

@@ -216,6 +216,7 @@ static void destroy(ptg_graph *g) {
     destroy_batch(g);
     destroy_streamed(g);
     release_sssp_split(g);
+    destroy_pr_bins(g);
     gfree(g, g->d_part);
     gfree(g, g->d_lpart);
     if (g->h_ring) (void)hipHostFree(g->h_ring);
@@ -244,6 +245,7 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
         return fail(PTG_ERR_INVALID_ARG, "graph_create: NaN sssp_pull or batch_pull");
     if (c.pr_block && c.pr_block != 256 && c.pr_block != 512 && c.pr_block != 1024)
         return fail(PTG_ERR_INVALID_ARG, "graph_create: pr_block is 0, 256, 512 or 1024");
+    if (c.pr_layout > PTG_PR_BINNED) return fail(PTG_ERR_INVALID_ARG, "graph_create: pr_layout is 0, 1 or 2");
     if (!(c.flags & PTG_FLAG_NO_VALIDATE)) {
         int rc = ptg_validate_csr(V, E, ptr, cols, wedges);
         if (rc) return rc;
@@ -587,6 +589,7 @@ int ptg_graph_set_config(ptg_graph *g, const ptg_config *c) {
         return fail(PTG_ERR_INVALID_ARG, "NaN sssp_pull or batch_pull");
     if (c->pr_block && c->pr_block != 256 && c->pr_block != 512 && c->pr_block != 1024)
         return fail(PTG_ERR_INVALID_ARG, "pr_block is 0, 256, 512 or 1024");
+    if (c->pr_layout > PTG_PR_BINNED) return fail(PTG_ERR_INVALID_ARG, "pr_layout is 0, 1 or 2");
     const bool replan = g->streamed && c->chunk_bytes != o.chunk_bytes;
     g->cfg = *c;
     g->pull_hist[0] = g->pull_hist[1] = ~0ull;  // SSSP pull rounds are looked for again

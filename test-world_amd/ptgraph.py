@@ -31,6 +31,7 @@ FLAG_TIMING = 0x2
 FLAG_NO_RELABEL = 0x4
 FLAG_STREAM = 0x8
 TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH, TUNE_STREAM, TUNE_PAGERANK = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20
+PR_AUTO, PR_PULL, PR_BINNED = 0, 1, 2  # Config.pr_layout
 Q_BFS, Q_SSSP, Q_CC, Q_PAGERANK = 0, 1, 2, 3
 
 BFS_UNREACHED = 0xFFFFFFFF  # ptGraph/main.cu:1068
@@ -81,6 +82,7 @@ class Config(ctypes.Structure):
         ("sssp_pull", ctypes.c_float),
         ("batch_pull", ctypes.c_float),
         ("pr_block", ctypes.c_uint32),
+        ("pr_layout", ctypes.c_uint32),
     ]
 
 

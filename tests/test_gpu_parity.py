@@ -198,16 +198,20 @@ def test_device_outputs_and_determinism(rmat16):
 
 def test_pagerank_bits_do_not_depend_on_the_work_split():
     """PageRank sums are exact fixed-point sums (63 fraction bits), so the
-    ranks are the same bits however the in-lists are divided: one workgroup,
-    an odd grid, every list cut into 64-edge hub chunks, the default bins, and
-    the allocation-free entry point's buffers. And within 1e-6 L1 of FP64."""
+    ranks are the same bits however the work is divided: the binned layout,
+    the in-list pull on one workgroup, an odd grid, every list cut into
+    64-edge hub chunks, the default bins, and the allocation-free entry
+    point's buffers. And within 1e-6 L1 of FP64."""
     csr = pg.gen_rmat(14, 16, seed=9)
     ptr, col = csr.node_pointers, csr.cols
     want = oracle.pagerank_f64(ptr, col, 0.85, 20)
     ref = None
-    for cfg in ({}, {"grid_blocks": 1}, {"grid_blocks": 37}, {"small_degree": 1, "large_degree": 64},
-                {"flags": pg.FLAG_NO_RELABEL}, {"pr_block": 512}, {"pr_block": 1024},
-                {"pr_block": 1024, "grid_blocks": 3}):
+    pull = pg.PR_PULL
+    for cfg in ({}, {"pr_layout": pull}, {"pr_layout": pull, "grid_blocks": 1},
+                {"pr_layout": pull, "grid_blocks": 37}, {"pr_layout": pull, "small_degree": 1, "large_degree": 64},
+                {"pr_layout": pull, "flags": pg.FLAG_NO_RELABEL}, {"flags": pg.FLAG_NO_RELABEL},
+                {"pr_layout": pull, "pr_block": 512}, {"pr_layout": pull, "pr_block": 1024},
+                {"pr_layout": pull, "pr_block": 1024, "grid_blocks": 3}, {"grid_blocks": 3}):
         say("case pagerank split %s" % cfg)
         with pg.Graph.from_csr(csr, weighted=False, config=pg.default_config(**cfg)) as g:
             r, _ = g.pagerank(0.85, 20)
@@ -221,6 +225,87 @@ def test_pagerank_bits_do_not_depend_on_the_work_split():
                 np.testing.assert_array_equal(buf.to_host(np.uint32, g.V), ref.view(np.uint32), err_msg=str(cfg))
             finally:
                 buf.free()
+
+
+def hub_graph(rng, V, hubs, leaves, extra):
+    """Symmetric graph whose first `hubs` vertices each join `leaves` random
+    others, plus `extra` random edges: the hub partition is cut into many
+    chunks."""
+    a = [np.repeat(np.arange(hubs), leaves), rng.integers(0, V, extra)]
+    b = [rng.integers(0, V, hubs * leaves), rng.integers(0, V, extra)]
+    s, d = np.concatenate(a), np.concatenate(b)
+    keep = s != d
+    s, d = np.concatenate([s[keep], d[keep]]), np.concatenate([d[keep], s[keep]])
+    pairs = np.unique(s.astype(np.int64) * V + d)
+    s, d = pairs // V, pairs % V
+    return np.searchsorted(s, np.arange(V)).astype(np.uint32), d.astype(np.uint32)
+
+
+def pagerank_layouts(ptr, col, cfg, iters, want=None):
+    """Ranks of the in-list pull and of the binned layout (forced and
+    automatic) on the same graph: the same bits, within 1e-6 L1 of FP64."""
+    if want is None:
+        want = oracle.pagerank_f64(ptr, col, 0.85, iters)
+    out = {}
+    for layout in (pg.PR_PULL, pg.PR_BINNED, pg.PR_AUTO):
+        with pg.Graph(ptr, col, config=pg.default_config(pr_layout=layout, **cfg)) as g:
+            before = g.device_bytes
+            r, _ = g.pagerank(0.85, iters)
+            grown = g.device_bytes - before
+            if layout == pg.PR_BINNED and iters and col.size:
+                assert grown >= 2 * col.size, grown  # its 16-bit edge entries are on the device
+            assert float(np.abs(r.astype(np.float64) - want).sum()) <= PR_L1, (layout, cfg)
+            out[layout] = r
+    for layout in (pg.PR_BINNED, pg.PR_AUTO):
+        np.testing.assert_array_equal(out[layout].view(np.uint32), out[pg.PR_PULL].view(np.uint32),
+                                      err_msg="layout %d %s" % (layout, cfg))
+    return out[pg.PR_PULL]
+
+
+@pytest.mark.parametrize("case", ["rmat16", "rmat15_no_relabel", "hubs", "directed", "sparse", "odd_sizes"])
+def test_pagerank_binned_equals_pull(case):
+    """Binned PageRank (destination partitions of 16384 vertices, LDS sums,
+    16-bit edge entries) against the in-list pull: the same bits for 0, 1, 7
+    and 20 iterations, on R-MAT with and without the degree order, a graph
+    whose hub partition is cut into many chunks, a directed graph (the binned
+    layout reads out-lists, the pull the transpose), a graph too sparse for
+    the automatic layout and sizes around the partition width."""
+    rng = np.random.default_rng(11)
+    cfg = {}
+    if case == "rmat16":
+        csr = pg.gen_rmat(16, 16, seed=3)
+        graphs = [(csr.node_pointers, csr.cols)]
+    elif case == "rmat15_no_relabel":
+        csr = pg.gen_rmat(15, 16, seed=4)
+        graphs = [(csr.node_pointers, csr.cols)]
+        cfg = {"flags": pg.FLAG_NO_RELABEL}
+    elif case == "hubs":
+        graphs = [hub_graph(rng, 70000, 40, 30000, 200000)]
+    elif case == "directed":
+        ptr, col, _ = random_directed(rng, 50000, 600000, 1)
+        graphs = [(ptr, col)]
+    elif case == "sparse":
+        graphs = [hub_graph(rng, 120000, 0, 0, 30000)]
+    else:
+        graphs = [hub_graph(rng, V, 3, 500, 4 * V) for V in (16383, 16384, 16385, 32769)]
+    for ptr, col in graphs:
+        say("case pagerank layouts %s V=%d E=%d" % (case, ptr.size, col.size))
+        for iters in (0, 1, 7, 20):
+            pagerank_layouts(ptr, col, cfg, iters)
+
+
+def test_pagerank_layout_tiny_and_dangling():
+    """One vertex, vertices without edges, and a graph where every vertex but
+    one is dangling: both layouts give the oracle's ranks, the same bits."""
+    cases = [
+        (np.zeros(1, np.uint32), np.zeros(0, np.uint32)),
+        (np.zeros(5, np.uint32), np.zeros(0, np.uint32)),
+        (np.array([0, 4, 4, 4, 4], np.uint32), np.array([1, 2, 3, 4], np.uint32)),  # star out of 0
+    ]
+    for ptr, col in cases:
+        say("case pagerank tiny V=%d E=%d" % (ptr.size, col.size))
+        for iters in (0, 1, 5):
+            pagerank_layouts(ptr, col, {}, iters)
 
 
 def test_pagerank_autotune_workgroup_size():

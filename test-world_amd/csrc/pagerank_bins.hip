@@ -60,6 +60,8 @@ constexpr uint32_t kAccWaves = kAccBlock / 64;
 constexpr uint32_t kTile = 64;            // vertices of a k_prb_fin unit whose rows the waves split
 constexpr uint32_t kRowsDirect = 4;       // at most this many rows: a thread per vertex adds them
 constexpr uint16_t kEntFirst = 0x8000, kEntNull = 0x4000, kEntDest = kPW - 1;
+constexpr uint64_t kNull4 = 0x4000400040004000ull;  // four null entries
+constexpr int kGroups = 4;                           // 256-entry groups per step of a k_prb_acc wave
 
 struct PrbChunk {
     uint64_t e0, e1;  // entries [e0, e1), within one partition
@@ -184,35 +186,48 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_acc(PrbArgs a) {
     for (uint64_t s = ch.e0 + (uint64_t)wave * kSeg; s < ch.e1; s += (uint64_t)kAccWaves * kSeg, j += kAccWaves) {
         const uint64_t se = s + kSeg < ch.e1 ? s + kSeg : ch.e1;
         uint32_t cur = a.seg_cur[j];  // pair holding the entry before s
-        // four entries per lane; lanes past the end read nulls (no pair start)
-        auto load4 = [&](uint64_t g) -> uint64_t {
-            const uint64_t e = g + 4ull * lane;
-            return e < se ? *reinterpret_cast<const uint64_t *>(a.ent + e) : 0x4000400040004000ull;
+        // kGroups groups of 256 entries per step, four per lane; lanes past
+        // the end read nulls (no pair start). The next step's entries are
+        // loaded before this step's bins are used.
+        auto load4 = [&](uint64_t e) -> uint64_t {
+            return e < se ? *reinterpret_cast<const uint64_t *>(a.ent + e) : kNull4;
         };
-        uint64_t w4 = load4(s);
-        for (uint64_t g = s; g < se; g += 256) {
-            const uint64_t nxt = g + 256 < se ? load4(g + 256) : 0x4000400040004000ull;
-            const uint32_t cnt = (uint32_t)((w4 >> 15) & 1) + (uint32_t)((w4 >> 31) & 1) +
-                                 (uint32_t)((w4 >> 47) & 1) + (uint32_t)((w4 >> 63) & 1);
-            const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2), b2 = ballot(cnt & 4);
-            uint32_t pr = cur + rank_below(b0) + 2 * rank_below(b1) + 4 * rank_below(b2);
-            cur += (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
-            // every lane's pair index is a valid bin (a null entry, or a lane
-            // past the end, repeats the pair before it); clamped regardless
-            float x[4];
+        uint64_t w[kGroups];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                pr += (uint32_t)(w4 >> (16 * i + 15)) & 1u;
-                x[i] = a.bin_in[min(pr, a.last_pair)];
+        for (int q = 0; q < kGroups; q++) w[q] = load4(s + 256ull * q + 4ull * lane);
+        for (uint64_t g = s; g < se; g += 256ull * kGroups) {
+            uint64_t nxt[kGroups];
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) nxt[q] = load4(g + 256ull * (kGroups + q) + 4ull * lane);
+            float x[kGroups][4];
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) {
+                const uint64_t w4 = w[q];
+                const uint32_t cnt = (uint32_t)((w4 >> 15) & 1) + (uint32_t)((w4 >> 31) & 1) +
+                                     (uint32_t)((w4 >> 47) & 1) + (uint32_t)((w4 >> 63) & 1);
+                const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2), b2 = ballot(cnt & 4);
+                uint32_t pr = cur + rank_below(b0) + 2 * rank_below(b1) + 4 * rank_below(b2);
+                cur += (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+                // every lane's pair index is a valid bin (a null entry, or a
+                // lane past the end, repeats the pair before it); clamped
+                // regardless
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    pr += (uint32_t)(w4 >> (16 * i + 15)) & 1u;
+                    x[q][i] = a.bin_in[min(pr, a.last_pair)];
+                }
             }
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t d = (uint32_t)(w4 >> (16 * i)) & 0xFFFFu;
-                if (!(d & kEntNull))
-                    __hip_atomic_fetch_add(&s_sum[d & kEntDest], fx63(x[i]), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-            w4 = nxt;
+            for (int q = 0; q < kGroups; q++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t d = (uint32_t)(w[q] >> (16 * i)) & 0xFFFFu;
+                    if (!(d & kEntNull))
+                        __hip_atomic_fetch_add(&s_sum[d & kEntDest], fx63(x[q][i]), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) w[q] = nxt[q];
         }
     }
     __syncthreads();
@@ -225,8 +240,8 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_acc(PrbArgs a) {
 // bins of r0 = 1/V instead.
 __global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
     __shared__ unsigned long long s_part[kAccWaves][kTile];
-    __shared__ uint32_t s_pb[kAccBlock + 1];
-    __shared__ float s_c[kAccBlock];
+    __shared__ uint32_t s_pb[kTile + 1];
+    __shared__ float s_c[kTile];
     __shared__ unsigned long long s_red[kAccWaves];
     const PrbUnit u = a.units[blockIdx.x];
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
@@ -245,10 +260,11 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
         }
     }
     unsigned long long dang = 0;
+    float c = 0.0f;
+    uint32_t kb = 0, ke = 0;
     if (threadIdx.x < u.nv) {
         const uint32_t v = u.v0 + threadIdx.x;
         const uint32_t od = a.ooff[v + 1] - a.ooff[v];
-        float c;
         if (a.init) {
             c = od ? a.inv_v / (float)od : 0.0f;
         } else {
@@ -257,12 +273,30 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
             if (!od) dang = fx63(r);
             c = od ? r / (float)od : 0.0f;
         }
-        s_c[threadIdx.x] = c;
-        s_pb[threadIdx.x] = a.pb[v];
+        kb = a.pb[v];
+        ke = a.pb[v + 1];
     }
-    if (threadIdx.x == 0) s_pb[u.nv] = a.pb[u.v0 + u.nv];
-    __syncthreads();
-    prb_scatter(a, u.nv, s_pb, s_c);
+    if (u.nrows > kRowsDirect) {
+        // hub units: many pairs per vertex, spread over the workgroup
+        if (threadIdx.x < u.nv) {
+            s_c[threadIdx.x] = c;
+            s_pb[threadIdx.x] = kb;
+        }
+        if (threadIdx.x == 0) s_pb[u.nv] = a.pb[u.v0 + u.nv];
+        __syncthreads();
+        prb_scatter(a, u.nv, s_pb, s_c);
+    } else {
+        // a thread per vertex, four pairs at a time
+        for (uint32_t k = kb; k < ke; k += 4) {
+            uint32_t at[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++)
+                if (k + i < ke) at[i] = a.pos[k + i];
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++)
+                if (k + i < ke) a.bin_out[at[i]] = c;
+        }
+    }
     if (a.init) return;
     dang = wave_sum_u64(dang);
     if (lane == 0) s_red[wave] = dang;

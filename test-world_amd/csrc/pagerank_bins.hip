@@ -62,6 +62,13 @@ constexpr uint32_t kRowsDirect = 4;       // at most this many rows: a thread pe
 constexpr uint16_t kEntFirst = 0x8000, kEntNull = 0x4000, kEntDest = kPW - 1;
 constexpr uint64_t kNull4 = 0x4000400040004000ull;  // four null entries
 constexpr int kGroups = 4;                           // 256-entry groups per step of a k_prb_acc wave
+// Chunk target: the entries / (2 x CUs); a partition of more entries is cut
+// into chunks of about that many. Finer chunks balance the launch better but
+// each writes a row of kPW sums (R-MAT-22: 1, 2, 3, 4 and 6 chunks per CU,
+// cutting above the target or twice it, all measured; 2 and the target were
+// the fastest).
+constexpr uint64_t kChunksPerCu = 2;
+constexpr uint64_t kChunkSplit = 1;
 
 struct PrbChunk {
     uint64_t e0, e1;  // entries [e0, e1), within one partition
@@ -414,17 +421,17 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
             }
         }
     });
-    // chunks: a partition of at most 2 x target entries is one chunk; larger
-    // ones are cut into runs of whole segments of about target entries. Each
-    // chunk has a row; the units of a partition add up its rows.
-    const uint64_t target = std::max<uint64_t>(16ull * kSeg, ent_off[P] / std::max<uint32_t>(1, num_cus));
+    // chunks: a partition of at most kChunkSplit x target entries is one
+    // chunk; larger ones are cut into runs of whole segments of about target
+    // entries. Each chunk has a row; the units of a partition add up its rows.
+    const uint64_t target = std::max<uint64_t>(16ull * kSeg, ent_off[P] / (kChunksPerCu * std::max<uint32_t>(1, num_cus)));
     out->chunks.clear();
     out->units.clear();
     out->n_rows = 0;
     for (uint32_t p = 0; p < P; p++) {
         const uint64_t n = ent_off[p + 1] - ent_off[p];
         const uint32_t row0 = out->n_rows;
-        if (n <= 2 * target) {
+        if (n <= kChunkSplit * target) {
             out->chunks.push_back(PrbChunk{ent_off[p], ent_off[p + 1], seg_off[p], out->n_rows++});
         } else {
             const uint64_t nseg = seg_off[p + 1] - seg_off[p];

@@ -16,7 +16,7 @@ roofline: the query kind with the largest summed main-kernel time in the
 warmup steps is the dominant kernel, and only its launches are timed (HIP
 events around each launch on the graph's stream) in the timed steps; its
 algorithmic bytes (BFS 12*V_r + 8*m_r, SSSP
-12*V_r + 12*m_r, CC 16*V + 8*E, PageRank 8*E + 20*V per launch) divided by
+12*V_r + 12*m_r, CC 16*V + 8*E, PageRank 8*E + 20*V per iteration) divided by
 its HIP-event-timed launch durations give `achieved`.
 
 cpu_baseline: the oracle's restatement of the reference's frontier loop
@@ -567,8 +567,11 @@ def main():
     per_kind = kind_stats(acc)
     per_kind_warm = kind_stats(wacc)
     # SSSP: a round is the push kernel plus, behind it, the pull kernel
+    # PageRank: an iteration is the binned accumulate kernel plus, behind it,
+    # the kernel that finishes the vertices (the layout the graph's automatic
+    # choice takes at R-MAT scale 22)
     kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false> + k_pull_round",
-                    "cc": "k_expand<HookRest>", "pagerank": "k_pr_gather"}
+                    "cc": "k_expand<HookRest>", "pagerank": "k_prb_acc + k_prb_fin"}
     achieved = per_kind[dom]["achieved_gbs"]
     roofline = None if args.no_timing else {
         "bound": "hbm",

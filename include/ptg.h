@@ -282,7 +282,7 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
 #define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
 #define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_sssp_grid, batch_chunk, batch_pull */
-#define PTG_TUNE_PAGERANK 0x20u   /* pr_block (timed on 3 iterations) */
+#define PTG_TUNE_PAGERANK 0x20u   /* pr_layout, then pr_block for the pull (timed on 3 iterations) */
 #define PTG_TUNE_STREAM 0x10u     /* streamed graphs only: chunk_bytes, stream_pinned, timed on BFS from
                                      the sources plus 2 PageRank iterations (symmetric graphs) */
 PTG_API int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t what, ptg_config *best);

@@ -176,9 +176,12 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G, 8 * G});
     }
     if (what & PTG_TUNE_PAGERANK) {
-        // threads per gather workgroup (the LDS hub stage grows with it)
+        // the layout (automatic, in-list pull, binned), then, for the pull,
+        // the threads per gather workgroup (its LDS hub stage grows with it)
+        if (!rc) rc = t.knob(kPagerank, &c.pr_layout, std::vector<uint32_t>{PTG_PR_PULL, PTG_PR_BINNED});
         if (!c.pr_block) c.pr_block = 256;
-        if (!rc) rc = t.knob(kPagerank, &c.pr_block, std::vector<uint32_t>{256, 512, 1024});
+        if (!rc && c.pr_layout == PTG_PR_PULL)
+            rc = t.knob(kPagerank, &c.pr_block, std::vector<uint32_t>{256, 512, 1024});
     }
     if ((what & PTG_TUNE_SSSP_BATCH) && g->weighted) {
         if (!rc) rc = t.knob(kSsspBatch, &c.batch_delta, std::vector<uint32_t>{32, 64, 128, 256, 1024, 4096});

@@ -309,20 +309,28 @@ def test_pagerank_layout_tiny_and_dangling():
 
 
 def test_pagerank_autotune_workgroup_size():
-    """The tuner's PageRank mode picks the gather's workgroup size (256, 512
-    or 1024 threads, the LDS hub stage growing with it); the ranks keep their
-    bits, and set_config refuses other sizes."""
+    """The tuner's PageRank mode picks the layout (in-list pull or binned)
+    and, for the pull, the gather's workgroup size (256, 512 or 1024 threads,
+    the LDS hub stage growing with it); the ranks keep their bits, and
+    set_config refuses other sizes and layouts."""
     csr = pg.gen_rmat(15, 16, seed=4)
     with pg.Graph.from_csr(csr, weighted=False) as g:
         want, _ = g.pagerank(0.85, 20)
         say("case pagerank autotune")
         c = g.autotune(np.array([0], np.uint32), what=pg.TUNE_PAGERANK)
+        assert c.pr_layout in (pg.PR_PULL, pg.PR_BINNED) and g.get_config().pr_layout == c.pr_layout
         assert c.pr_block in (256, 512, 1024) and g.get_config().pr_block == c.pr_block
         np.testing.assert_array_equal(g.pagerank(0.85, 20)[0].view(np.uint32), want.view(np.uint32))
+        for layout in (pg.PR_PULL, pg.PR_BINNED, pg.PR_AUTO):
+            g.tune(pr_layout=layout)
+            np.testing.assert_array_equal(g.pagerank(0.85, 20)[0].view(np.uint32), want.view(np.uint32))
         for bad in (128, 300, 2048):
             with pytest.raises(pg.PtgError) as e:
                 g.tune(pr_block=bad)
             assert e.value.code == pg.ERR_INVALID_ARG
+        with pytest.raises(pg.PtgError) as e:
+            g.tune(pr_layout=3)
+        assert e.value.code == pg.ERR_INVALID_ARG
 
 
 def random_directed(rng, V, E, wmax):

@@ -252,7 +252,8 @@ PTG_API int ptg_pagerank(ptg_graph *g, float damping, int iterations, float *ran
  * one fixed 64 MiB device array per GPU, so V is limited to about 8.3 M
  * (PTG_ERR_BUDGET beyond). ranks_out must be device memory
  * (PTG_ERR_INVALID_ARG otherwise); a streamed graph, or a graph that is not
- * symmetric and has not yet run ptg_pagerank (which builds its in-edges),
+ * symmetric and has not yet run ptg_pagerank with pr_layout PTG_PR_PULL
+ * (the in-list pull builds its in-edges; the binned layout does not),
  * gives PTG_ERR_UNSUPPORTED. Per-launch timing is never recorded. Results are
  * bit-identical to ptg_pagerank's. Every call on one GPU shares that GPU's
  * array: calls that overlap on one GPU race. */

@@ -176,8 +176,11 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kBfsBatch, &c.batch_grid, std::vector<uint32_t>{G, 2 * G, 4 * G, 8 * G});
     }
     if (what & PTG_TUNE_PAGERANK) {
-        // the layout (automatic, in-list pull, binned), then, for the pull,
-        // the threads per gather workgroup (its LDS hub stage grows with it)
+        // the layout (in-list pull or binned; the search starts from an
+        // explicit one so an automatic layout never stays set), then, for the
+        // pull, the threads per gather workgroup (its LDS hub stage grows
+        // with it)
+        if (c.pr_layout == PTG_PR_AUTO) c.pr_layout = PTG_PR_BINNED;
         if (!rc) rc = t.knob(kPagerank, &c.pr_layout, std::vector<uint32_t>{PTG_PR_PULL, PTG_PR_BINNED});
         if (!c.pr_block) c.pr_block = 256;
         if (!rc && c.pr_layout == PTG_PR_PULL)

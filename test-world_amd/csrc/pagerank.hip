@@ -433,13 +433,15 @@ uint64_t pr_scratch_need(const ptg_graph *g) {
 int run_pagerank_noalloc(ptg_graph *g, float damping, int iters, float *ranks, ptg_stats *st) {
     int rc;
     // in-edges without allocating: the graph's own lists when it is
-    // symmetric, else a transpose an earlier ptg_pagerank built
+    // symmetric, else a transpose an earlier in-list pull (ptg_pagerank
+    // with pr_layout PTG_PR_PULL) built; the binned layout builds none
     if (!g->have_in) {
         int sym = g->symmetric;
         if (sym < 0) return fail(PTG_ERR_UNSUPPORTED, "pagerank_noalloc: symmetry not yet known");
         if (!sym)
             return fail(PTG_ERR_UNSUPPORTED,
-                        "pagerank_noalloc: graph is not symmetric and has no in-edges yet (run ptg_pagerank once)");
+                        "pagerank_noalloc: graph is not symmetric and has no in-edges yet (run ptg_pagerank "
+                        "once with pr_layout PTG_PR_PULL)");
         if ((rc = ensure_in_edges(g))) return rc;  // aliases the out-lists, no allocation
     }
     const uint64_t need = pr_scratch_need(g);

@@ -67,7 +67,14 @@ def test_noalloc_directed_needs_in_edges():
             with pytest.raises(pg.PtgError) as e:
                 g.pagerank_noalloc(buf)
             assert e.value.code == pg.ERR_UNSUPPORTED
-            want, _ = g.pagerank(0.85, 20)  # builds the in-edges
+            g.tune(pr_layout=pg.PR_BINNED)
+            want, _ = g.pagerank(0.85, 20)  # the binned layout builds no in-edges
+            with pytest.raises(pg.PtgError) as e:
+                g.pagerank_noalloc(buf)
+            assert e.value.code == pg.ERR_UNSUPPORTED
+            g.tune(pr_layout=pg.PR_PULL)
+            pulled, _ = g.pagerank(0.85, 20)  # the in-list pull builds the in-edges
+            np.testing.assert_array_equal(pulled.view(np.uint32), want.view(np.uint32))
             g.pagerank_noalloc(buf)
             np.testing.assert_array_equal(buf.to_host(np.uint32, V), want.view(np.uint32))
         finally:

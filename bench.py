@@ -203,10 +203,11 @@ def serving_leg(args, pg, np, cores, log):
     tune_rng = np.random.default_rng(5)
     for sc in scales:
         c = pg.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=1, threads=cores).add_weights(1, 64, seed=2, threads=cores)
-        # PageRank by the in-list pull: its query state (two contribution
-        # arrays) is a fifteenth of the binned layout's, which under the
-        # state budget would evict the batch state of the other graphs
-        gr = pg.Graph.from_csr(c, config=pg.default_config(pr_layout=pg.PR_PULL))
+        # PageRank by the in-list pull and SSSP without sweep rounds: the
+        # binned layout both would use is about fifteen times PageRank's pull
+        # state, and under the state budget it would evict the batch state of
+        # the other graphs
+        gr = pg.Graph.from_csr(c, config=pg.default_config(pr_layout=pg.PR_PULL, sssp_sweep=-1.0))
         if not args.no_tune:
             # the autotuner picks each graph's knobs for both paths (single
             # queries and batches) on 64 of its own sources
@@ -577,11 +578,12 @@ def main():
 
     per_kind = kind_stats(acc)
     per_kind_warm = kind_stats(wacc)
-    # SSSP: a round is the push kernel plus, behind it, the pull kernel
+    # SSSP: a round is the push kernel plus, behind it, the pull kernel and
+    # the two sweep kernels (whichever of them expands the round)
     # PageRank: an iteration is the binned accumulate kernel plus, behind it,
     # the kernel that finishes the vertices (the layout the graph's automatic
     # choice takes at R-MAT scale 22)
-    kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false> + k_pull_round",
+    kernel_names = {"bfs": "k_bfs_round", "sssp": "k_relax_round<true, false> + k_pull_round + k_sw_acc + k_sw_fin",
                     "cc": "k_expand<HookRest>", "pagerank": "k_prb_acc + k_prb_fin"}
     achieved = per_kind[dom]["achieved_gbs"]
     roofline = None if args.no_timing else {

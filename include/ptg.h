@@ -125,6 +125,14 @@ typedef struct ptg_config {
                                      (source, destination partition) pairs come to at most half
                                      the edges, else the in-list pull), PTG_PR_PULL or
                                      PTG_PR_BINNED. Every layout gives the same bits */
+    float sssp_sweep;             /* SSSP where pulls are possible, weights <= 255 and the binned
+                                     layout pays (as for PageRank): a large round relaxes every
+                                     edge through the binned layout (a byte of weight per edge,
+                                     an LDS minimum per destination partition) when the round's
+                                     push (its out-edges) and pull (in-edges x sssp_pull) would
+                                     both scan more than E x sssp_sweep edges; the distances are
+                                     the same. 0 = auto (0.25), < 0 = never (the layout is then
+                                     not built for SSSP) */
 } ptg_config;
 
 /* ptg_config.pr_layout */
@@ -144,11 +152,12 @@ typedef struct ptg_stats {
     uint64_t host_link_ns;    /* time of those copies */
     uint64_t main_kernel_ns;  /* PTG_FLAG_TIMING: summed HIP-event time of the query's
                                  main kernel (frontier expansion / PageRank sweep; SSSP:
-                                 the push and the pull kernel of every round) */
+                                 the push, pull and sweep kernels of every round) */
     uint32_t main_launches;   /* PTG_FLAG_TIMING: launches of that kernel (SSSP: rounds) */
     uint32_t chunk_loads;     /* PTG_FLAG_STREAM: chunks copied from the host during the query */
     uint32_t chunk_hits;      /* PTG_FLAG_STREAM: chunk uses served from the device cache */
     uint32_t pull_rounds;     /* SSSP: rounds that pulled over in-edges (sssp_pull) */
+    uint32_t sweep_rounds;    /* SSSP: rounds that swept every edge (sssp_sweep) */
 } ptg_stats;
 
 typedef struct ptg_graph ptg_graph;
@@ -280,7 +289,8 @@ PTG_API int ptg_sssp_batch(ptg_graph *g, const uint32_t *srcs, int n, uint32_t *
  * speed does. No reference counterpart (the reference hard-codes 56 x 1024
  * threads, its partition fraction and fragment size, main.cu:66, :318-319). */
 #define PTG_TUNE_BFS 0x1u         /* td_to_bu, bu_to_td */
-#define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree */
+#define PTG_TUNE_SSSP 0x2u        /* sssp_delta, sssp_defer_window, sssp_defer_degree, sssp_pull,
+                                     sssp_sweep (unless set below 0) */
 #define PTG_TUNE_BFS_BATCH 0x4u   /* batch_td_to_bu, batch_grid */
 #define PTG_TUNE_SSSP_BATCH 0x8u  /* batch_delta, batch_sssp_grid, batch_chunk, batch_pull */
 #define PTG_TUNE_PAGERANK 0x20u   /* pr_layout, then pr_block for the pull (timed on 3 iterations) */

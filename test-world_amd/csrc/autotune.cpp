@@ -168,6 +168,11 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kSssp, &c.sssp_defer_degree, std::vector<uint32_t>{0, 32, 128, 256, 0xFFFFFFFFu});
         // 0 = pull when the open in-edges are fewer than the frontier's edges; -1 = never
         if (!rc) rc = t.knob(kSssp, &c.sssp_pull, std::vector<float>{0, 0.5f, 2, -1});
+        // 0 = sweep a round whose push and pull would each scan > E / 4; -1 =
+        // never (a graph set to never keeps it: its caller keeps the binned
+        // layout's memory free)
+        if (!rc && c.sssp_pull >= 0 && c.sssp_sweep >= 0)
+            rc = t.knob(kSssp, &c.sssp_sweep, std::vector<float>{0, 0.15f, 0.4f, -1});
         if (!rc && !c.sssp_delta) release_sssp_split(g);  // the light/heavy CSRs tried above
         if (!rc && c.sssp_pull < 0) release_sssp_pull(g);
     }

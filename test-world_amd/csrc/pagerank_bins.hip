@@ -41,6 +41,21 @@
 // (the edges u -> v themselves, so a directed graph needs no transpose) and
 // kept in host memory, so releasing the device copy and uploading it again
 // is a plain copy.
+//
+// SSSP sweep rounds use the same layout with one more array, a weight byte
+// per entry (graphs whose weights are at most 255) and the source of every
+// pair in bin order. A sweep relaxes every edge u -> v with the value u had
+// when the round began: k_sw_acc reads each entry's source value through its
+// pair's source (in a partition the pairs' sources ascend, so the reads of a
+// wave share lines where the sources are dense), takes per chunk the
+// minimum of bin + weight over the entries of each destination in LDS (a
+// 32-bit LDS atomicMin) and writes the chunk's row of minima, and k_sw_fin
+// takes each vertex's minimum over its partition's rows; a vertex whose
+// minimum is below its value takes it and joins the next frontier, as a
+// pushed vertex does. Relaxing an edge out of a vertex outside the frontier
+// changes nothing (its edges were relaxed at its current value), so a sweep
+// round reaches what a push round would, by the same strict-< rule, and the
+// distances are the same.
 #include <string.h>
 
 #include <algorithm>
@@ -48,7 +63,7 @@
 #include <thread>
 #include <vector>
 
-#include "device_util.h"
+#include "rounds.h"
 
 namespace ptg {
 
@@ -86,6 +101,9 @@ struct PrbUnit {
 // Host copy of the layout.
 struct PrBinsHost {
     std::vector<uint16_t> ent;
+    std::vector<uint8_t> went;  // SSSP sweeps: the weight of every entry (0 for nulls); empty when not built
+    std::vector<uint32_t> bsrc; // SSSP sweeps: the source of every pair, in bin order
+    bool has_w = false;
     std::vector<uint32_t> pos, pb, seg_cur;
     std::vector<PrbChunk> chunks;
     std::vector<PrbUnit> units;
@@ -99,6 +117,8 @@ struct PrBins {
     bool usable = false;  // auto layout: false when pairs would exceed half the edges
     // device copy
     uint16_t *ent = nullptr;
+    uint8_t *went = nullptr;
+    uint32_t *bsrc = nullptr;
     uint32_t *pos = nullptr, *pb = nullptr, *seg_cur = nullptr;
     PrbChunk *chunks = nullptr;
     PrbUnit *units = nullptr;
@@ -333,8 +353,8 @@ static void run_ranges(const std::vector<uint32_t> &vb, F f) {
 
 // Builds the layout from CSR out-lists (each list ascending). Returns false,
 // with nothing built, when `require_gain` and the pairs exceed half the edges.
-static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint32_t *col, uint32_t num_cus,
-                         bool require_gain, PrBinsHost *out) {
+static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint32_t *col, const uint32_t *w,
+                         uint32_t num_cus, bool require_gain, PrBinsHost *out) {
     const uint32_t P = (V + kPW - 1) >> kPWBits;
     // thread ranges of about equal edges
     const int T = (int)std::max<uint64_t>(1, std::min<uint64_t>(build_threads(), (uint64_t)V / 4096 + 1));
@@ -397,6 +417,14 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
     out->n_pairs = pairs;
     out->pos.assign(pairs ? pairs : 1, 0);
     out->ent.assign(ent_off[P] ? ent_off[P] : 4, kEntNull);
+    out->has_w = w != nullptr;
+    if (w) {
+        out->went.assign(out->ent.size(), 0);
+        out->bsrc.assign(out->pos.size(), 0);
+    } else {
+        std::vector<uint8_t>().swap(out->went);
+        std::vector<uint32_t>().swap(out->bsrc);
+    }
     out->seg_cur.assign(seg_off[P] ? seg_off[P] : 1, 0);
     // pass 2: entries, pair positions, segment starts
     run_ranges(vb, [&](int t, uint32_t b, uint32_t e) {
@@ -412,10 +440,12 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
                 if (first) {
                     pp = (uint32_t)pcur[p]++;
                     out->pos[kp++] = pp;
+                    if (w) out->bsrc[pp] = u;
                     prev = p;
                 }
                 const uint64_t x = ecur[p]++;
                 out->ent[x] = (uint16_t)((v & kEntDest) | (first ? kEntFirst : 0));
+                if (w) out->went[x] = (uint8_t)w[k];
                 const uint64_t rel = x - ent_off[p];
                 if (rel % kSeg == 0) out->seg_cur[seg_off[p] + rel / kSeg] = pp - (first ? 1u : 0u);
             }
@@ -457,7 +487,8 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
 static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
 
 static uint64_t device_bytes_of(const PrBinsHost &h, uint32_t V) {
-    return al256(2 * h.ent.size()) + al256(4 * h.pos.size()) + al256(4ull * ((uint64_t)V + 1)) +
+    return al256(2 * h.ent.size()) + al256(h.went.size()) + al256(4 * h.bsrc.size()) + al256(4 * h.pos.size()) +
+           al256(4ull * ((uint64_t)V + 1)) +
            al256(4 * h.seg_cur.size()) + al256(sizeof(PrbChunk) * h.chunks.size()) +
            al256(sizeof(PrbUnit) * h.units.size()) + 2 * al256(4 * h.pos.size()) +
            al256(8ull * kPW * h.n_rows) + al256(8 * (2 * h.units.size() + 1));
@@ -465,6 +496,8 @@ static uint64_t device_bytes_of(const PrBinsHost &h, uint32_t V) {
 
 static void free_device(ptg_graph *g, PrBins *b) {
     gfree(g, b->ent);
+    gfree(g, b->went);
+    gfree(g, b->bsrc);
     gfree(g, b->pos);
     gfree(g, b->pb);
     gfree(g, b->seg_cur);
@@ -475,6 +508,8 @@ static void free_device(ptg_graph *g, PrBins *b) {
     gfree(g, b->rows);
     gfree(g, b->part);
     b->ent = nullptr;
+    b->went = nullptr;
+    b->bsrc = nullptr;
     b->pos = b->pb = b->seg_cur = nullptr;
     b->chunks = nullptr;
     b->units = nullptr;
@@ -494,7 +529,9 @@ static int upload(ptg_graph *g, PrBins *b) {
         (rc = galloc(g, (void **)&b->bins[0], 4 * h.pos.size())) ||
         (rc = galloc(g, (void **)&b->bins[1], 4 * h.pos.size())) ||
         (rc = galloc(g, (void **)&b->rows, 8ull * kPW * h.n_rows)) ||
-        (rc = galloc(g, (void **)&b->part, 8 * (2 * h.units.size() + 1)))) {
+        (rc = galloc(g, (void **)&b->part, 8 * (2 * h.units.size() + 1))) ||
+        (h.has_w && (rc = galloc(g, (void **)&b->went, h.went.size()))) ||
+        (h.has_w && (rc = galloc(g, (void **)&b->bsrc, 4 * h.bsrc.size())))) {
         free_device(g, b);
         return rc;
     }
@@ -503,7 +540,9 @@ static int upload(ptg_graph *g, PrBins *b) {
         (rc = copy_sync(g, b->pb, h.pb.data(), 4 * h.pb.size(), hipMemcpyHostToDevice)) ||
         (rc = copy_sync(g, b->seg_cur, h.seg_cur.data(), 4 * h.seg_cur.size(), hipMemcpyHostToDevice)) ||
         (rc = copy_sync(g, b->chunks, h.chunks.data(), sizeof(PrbChunk) * h.chunks.size(), hipMemcpyHostToDevice)) ||
-        (rc = copy_sync(g, b->units, h.units.data(), sizeof(PrbUnit) * h.units.size(), hipMemcpyHostToDevice))) {
+        (rc = copy_sync(g, b->units, h.units.data(), sizeof(PrbUnit) * h.units.size(), hipMemcpyHostToDevice)) ||
+        (h.has_w && (rc = copy_sync(g, b->went, h.went.data(), h.went.size(), hipMemcpyHostToDevice))) ||
+        (h.has_w && (rc = copy_sync(g, b->bsrc, h.bsrc.data(), 4 * h.bsrc.size(), hipMemcpyHostToDevice)))) {
         free_device(g, b);
         return rc;
     }
@@ -511,16 +550,22 @@ static int upload(ptg_graph *g, PrBins *b) {
     return PTG_OK;
 }
 
-// Builds the layout from the device out-lists.
-static int build_from_device(ptg_graph *g, bool require_gain, bool *usable) {
+// Builds the layout from the device out-lists (with a weight byte per entry
+// when with_w; the weights must be at most 255).
+static int build_from_device(ptg_graph *g, bool require_gain, bool *usable, bool with_w = false) {
     const uint32_t V = g->V;
     const uint64_t E = g->E;
-    std::vector<uint32_t> off((size_t)V + 1), col(E ? E : 1);
+    std::vector<uint32_t> off((size_t)V + 1), col(E ? E : 1), w;
     int rc;
     if ((rc = copy_sync(g, off.data(), g->out.off, 4ull * ((uint64_t)V + 1), hipMemcpyDeviceToHost)) ||
         (E && (rc = copy_sync(g, col.data(), g->out.col, 4 * E, hipMemcpyDeviceToHost))))
         return rc;
-    *usable = build_layout(V, E, off.data(), col.data(), (uint32_t)std::max(1, g->num_cus), require_gain, &g->prb->h);
+    if (with_w) {
+        w.resize(E ? E : 1);
+        if (E && (rc = copy_sync(g, w.data(), g->d_w, 4 * E, hipMemcpyDeviceToHost))) return rc;
+    }
+    *usable = build_layout(V, E, off.data(), col.data(), with_w ? w.data() : nullptr,
+                           (uint32_t)std::max(1, g->num_cus), require_gain, &g->prb->h);
     return PTG_OK;
 }
 
@@ -632,6 +677,235 @@ int run_pagerank_bins(ptg_graph *g, float damping, int iters, float *ranks, ptg_
         st->edges_traversed = g->E * (uint64_t)iters;
         if ((rc = timer.collect(st))) return rc;
     }
+    return PTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SSSP sweep rounds.
+// ---------------------------------------------------------------------------
+
+struct SwArgs {
+    RoundEnv e;               // the round: ring entry e.r in, e.r + 1 out, output flags
+    const uint16_t *ent;
+    const uint8_t *went;
+    const uint32_t *seg_cur;
+    const PrbChunk *chunks;
+    const PrbUnit *units;
+    const uint32_t *bsrc;     // the source of each pair, in bin order
+    uint32_t *rows;           // per chunk, the minima of its partition's 16384 destinations
+    uint32_t *val;
+    uint32_t sent;            // value of unreached vertices (their bins relax nothing)
+    uint32_t last_pair;
+    uint32_t n_vert;
+    uint32_t n_units;
+    float factor;             // the round's pull factor (0: no pull kernel this round)
+    uint64_t min_edges;       // the round's sweep threshold
+};
+
+// Every kernel of a sweep checks first whether round e.r sweeps, with the
+// decision its push and pull kernels make.
+__device__ __forceinline__ bool sw_active(const SwArgs &a) {
+    const RoundCtr *in = a.e.ring + a.e.r % kRing;
+    uint32_t found = 0;
+    uint64_t edges = 0;
+#pragma unroll
+    for (int i = 0; i < kShards; i++) {
+        found += in->found_sh[i];
+        edges += in->edges_sh[i];
+    }
+    return round_sweeps(in, found, edges, a.e.qcap, a.factor, a.min_edges);
+}
+
+// One workgroup per chunk: the minimum of bin + weight over the chunk's
+// entries of every destination of its partition, in LDS, written as the
+// chunk's row (0xFFFFFFFF where no entry reaches a destination). The walk is
+// k_prb_acc's: four 256-entry groups in flight per wave, each entry's pair
+// found from the segment's first pair and the entries' first-of-pair bits.
+__global__ __launch_bounds__(kAccBlock) void k_sw_acc(SwArgs a) {
+    __shared__ uint32_t s_min[kPW];
+    if (!sw_active(a)) return;
+    const PrbChunk ch = a.chunks[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < kPW; i += kAccBlock) s_min[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    uint32_t j = ch.seg0 + wave;
+    for (uint64_t s = ch.e0 + (uint64_t)wave * kSeg; s < ch.e1; s += (uint64_t)kAccWaves * kSeg, j += kAccWaves) {
+        const uint64_t se = s + kSeg < ch.e1 ? s + kSeg : ch.e1;
+        uint32_t cur = a.seg_cur[j];  // pair holding the entry before s
+        auto load4 = [&](uint64_t e) -> uint64_t {
+            return e < se ? *reinterpret_cast<const uint64_t *>(a.ent + e) : kNull4;
+        };
+        auto loadw = [&](uint64_t e) -> uint32_t {
+            return e < se ? *reinterpret_cast<const uint32_t *>(a.went + e) : 0u;
+        };
+        uint64_t w[kGroups];
+        uint32_t wt[kGroups];
+#pragma unroll
+        for (int q = 0; q < kGroups; q++) {
+            w[q] = load4(s + 256ull * q + 4ull * lane);
+            wt[q] = loadw(s + 256ull * q + 4ull * lane);
+        }
+        for (uint64_t g = s; g < se; g += 256ull * kGroups) {
+            uint64_t nxt[kGroups];
+            uint32_t nwt[kGroups];
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) {
+                nxt[q] = load4(g + 256ull * (kGroups + q) + 4ull * lane);
+                nwt[q] = loadw(g + 256ull * (kGroups + q) + 4ull * lane);
+            }
+            uint32_t src[kGroups][4], x[kGroups][4];
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) {
+                const uint64_t w4 = w[q];
+                const uint32_t cnt = (uint32_t)((w4 >> 15) & 1) + (uint32_t)((w4 >> 31) & 1) +
+                                     (uint32_t)((w4 >> 47) & 1) + (uint32_t)((w4 >> 63) & 1);
+                const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2), b2 = ballot(cnt & 4);
+                uint32_t pr = cur + rank_below(b0) + 2 * rank_below(b1) + 4 * rank_below(b2);
+                cur += (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    pr += (uint32_t)(w4 >> (16 * i + 15)) & 1u;
+                    src[q][i] = a.bsrc[min(pr, a.last_pair)];
+                }
+            }
+            // every source's value read before any is used
+#pragma unroll
+            for (int q = 0; q < kGroups; q++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) x[q][i] = a.val[src[q][i]];
+#pragma unroll
+            for (int q = 0; q < kGroups; q++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t d = (uint32_t)(w[q] >> (16 * i)) & 0xFFFFu;
+                    const uint32_t xv = x[q][i];
+                    if (!(d & kEntNull) && xv < a.sent)
+                        __hip_atomic_fetch_min(&s_min[d & kEntDest], xv + ((wt[q] >> (8 * i)) & 0xFFu),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) {
+                w[q] = nxt[q];
+                wt[q] = nwt[q];
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t *row = a.rows + (uint64_t)ch.row * kPW;
+    for (uint32_t i = threadIdx.x; i < kPW; i += kAccBlock) row[i] = s_min[i];
+}
+
+// One workgroup per unit: each vertex's minimum over its partition's rows;
+// a vertex whose minimum is below its value takes it and goes to the output
+// flags. As in a pull round, the output entry gets only `any` and the
+// smallest value put into it; the next pre-pass counts the flags.
+__global__ __launch_bounds__(kAccBlock) void k_sw_fin(SwArgs a) {
+    __shared__ uint32_t s_part[kAccWaves][kTile];
+    __shared__ uint32_t s_any[kAccWaves], s_fmin[kAccWaves];
+    if (!sw_active(a)) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    bool improved = false;
+    uint32_t fmin_inv = 0;
+    // a fixed grid walks the units (an idle launch, in a round that does not
+    // sweep, then costs a few workgroups, not one per unit)
+    for (uint32_t ui = blockIdx.x; ui < a.n_units; ui += gridDim.x) {
+        const PrbUnit u = a.units[ui];
+        const uint32_t off0 = u.v0 & kEntDest;
+        uint32_t m = 0xFFFFFFFFu;
+        if (u.nrows > kRowsDirect) {
+            for (uint32_t r = wave; r < u.nrows; r += kAccWaves)
+                m = min(m, a.rows[(uint64_t)(u.row0 + r) * kPW + off0 + lane]);
+            s_part[wave][lane] = m;
+            __syncthreads();
+            m = 0xFFFFFFFFu;
+            if (threadIdx.x < kTile)
+                for (uint32_t w = 0; w < kAccWaves; w++) m = min(m, s_part[w][threadIdx.x]);
+            __syncthreads();  // s_part is rewritten by the next unit
+        } else if (threadIdx.x < u.nv) {
+            for (uint32_t r = 0; r < u.nrows; r++) m = min(m, a.rows[(uint64_t)(u.row0 + r) * kPW + off0 + threadIdx.x]);
+        }
+        if (threadIdx.x < u.nv) {
+            const uint32_t v = u.v0 + threadIdx.x;
+            if (m < a.val[v]) {
+                a.val[v] = m;
+                a.e.flout[v] = 1;
+                improved = true;
+                fmin_inv = max(fmin_inv, ~m);
+            }
+        }
+    }
+    const uint32_t any = ballot(improved) ? 1u : 0u;
+    const uint32_t fm = wave_max_u32(fmin_inv);
+    if (lane == 0) {
+        s_any[wave] = any;
+        s_fmin[wave] = fm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tm = 0;
+        for (uint32_t w = 0; w < kAccWaves; w++) {
+            ta |= s_any[w];
+            tm = max(tm, s_fmin[w]);
+        }
+        RoundCtr *out = a.e.ring + (a.e.r + 1) % kRing;
+        if (ta) out->any = 1u;
+        if (tm && tm > out->fmin_inv) atomicMax(&out->fmin_inv, tm);
+        if (blockIdx.x == 0) (a.e.ring + a.e.r % kRing)->swept = 1u;
+    }
+}
+
+int sssp_sweep_prepare(ptg_graph *g, bool *ok) {
+    *ok = false;
+    if (!g->weighted || g->cfg.sssp_sweep < 0 || g->wmax > 255 || g->E == 0) return PTG_OK;
+    if (!g->prb) g->prb = new PrBins();
+    PrBins *b = g->prb;
+    int rc;
+    if (!b->built || !b->h.has_w) {
+        // (re)built with the weight bytes; the entries and pairs come out the same
+        free_device(g, b);
+        b->built = false;
+        if ((rc = build_from_device(g, true, &b->usable, true))) return rc;
+        b->built = true;
+    }
+    if (!b->usable) return PTG_OK;
+    if (!b->on_device && (rc = upload(g, b))) return rc;
+    *ok = true;
+    return PTG_OK;
+}
+
+uint64_t sssp_sweep_need(const ptg_graph *g) {
+    if (!g->weighted || g->cfg.sssp_sweep < 0 || (g->auto_window && g->wmax > 255)) return 0;
+    const PrBins *b = g->prb;
+    if (b && b->built && b->h.has_w) return b->usable && !b->on_device ? device_bytes_of(b->h, g->V) : 0;
+    // not built with weights yet: the layout's bound plus a byte per entry
+    return pr_bins_need(g) + ((g->E + 4ull * ((g->V + kPW - 1) / kPW) + 255) & ~255ull);
+}
+
+int sssp_sweep_launch(ptg_graph *g, const RoundEnv &env, uint32_t *val, uint32_t sent, float pull_factor,
+                      uint64_t min_edges, LaunchTimer &timer) {
+    const PrBins *b = g->prb;
+    if (!b || !b->on_device || !b->h.has_w) return fail(PTG_ERR_UNSUPPORTED, "sweep round without its layout");
+    SwArgs a;
+    memset(&a, 0, sizeof a);
+    a.e = env;
+    a.ent = b->ent;
+    a.went = b->went;
+    a.seg_cur = b->seg_cur;
+    a.chunks = b->chunks;
+    a.units = b->units;
+    a.bsrc = b->bsrc;
+    a.rows = reinterpret_cast<uint32_t *>(b->rows);
+    a.val = val;
+    a.sent = sent;
+    a.last_pair = (uint32_t)(b->h.n_pairs ? b->h.n_pairs - 1 : 0);
+    a.n_vert = g->V;
+    a.n_units = (uint32_t)b->h.units.size();
+    a.factor = pull_factor;
+    a.min_edges = min_edges;
+    const uint32_t nc = (uint32_t)b->h.chunks.size(), nu = (uint32_t)b->h.units.size();
+    timer.launch_rider(k_sw_acc, dim3(nc), dim3(kAccBlock), a);
+    timer.launch_rider(k_sw_fin, dim3(std::min(nu, 2u * (uint32_t)std::max(1, g->num_cus))), dim3(kAccBlock), a);
+    PTG_HIP(hipGetLastError());
     return PTG_OK;
 }
 

@@ -57,7 +57,8 @@ struct RoundCtr {
     uint32_t ndefer;                        // SSSP: 1 if the pre-pass deferred any vertex
     uint32_t ucount;                        // SSSP: 1 if the pre-pass filled uedges_sh
     uint32_t pulled;                        // SSSP: 1 if the round reading this entry pulled
-    uint32_t pad[3];
+    uint32_t swept;                         // SSSP: 1 if the round reading this entry swept every edge
+    uint32_t pad[2];
     uint32_t found_sh[kShards];             // frontier vertices (any degree)
     uint32_t far_sh[kShards];               // SSSP: vertices added to / kept in the far set
     unsigned long long edges_sh[kShards];   // sum of out-degrees of the frontier
@@ -225,6 +226,11 @@ struct ptg_graph {
     // known) and the queries run, for the pull kernel's launch window.
     uint64_t pull_hist[2] = {~0ull, ~0ull};
     uint32_t pull_queries = 0;
+    // The same for the binned sweep rounds (rounds of the last query that
+    // swept, and those of the last two queries).
+    uint32_t swept_rounds = 0;
+    uint64_t swept_mask = 0;
+    uint64_t sweep_hist[2] = {~0ull, ~0ull};
     // Rounds of the last BFS [0], SSSP [1] and label-propagation [2] query
     // (0 = none yet): the next query of the kind launches that many rounds + 1
     // before its first readback.
@@ -274,6 +280,20 @@ uint64_t pr_bins_need(const ptg_graph *g);
 void release_pr_bins(ptg_graph *g);
 void destroy_pr_bins(ptg_graph *g);
 int pr_bins_info(const ptg_graph *g, uint64_t *pairs, uint32_t *chunks, uint32_t *rows);
+// SSSP sweep rounds over the binned layout (pagerank_bins.hip). *ok = false
+// when the graph cannot sweep (weights above 255, or the automatic layout
+// would not pay); else the layout, with a weight byte per entry and a source
+// per pair, is on the device. sssp_sweep_need: device bytes that call would
+// still allocate.
+int sssp_sweep_prepare(ptg_graph *g, bool *ok);
+uint64_t sssp_sweep_need(const ptg_graph *g);
+// Launches the two kernels of a possible sweep of round env.r behind its
+// push (and pull) kernels; each returns at once unless the round sweeps
+// (round_sweeps with the same factor and edge threshold).
+struct RoundEnv;
+struct LaunchTimer;
+int sssp_sweep_launch(ptg_graph *g, const RoundEnv &env, uint32_t *val, uint32_t sent, float pull_factor,
+                      uint64_t min_edges, LaunchTimer &timer);
 // PageRank with its per-iteration buffers in a fixed per-GPU device array of
 // kPrScratchBytes (pagerank.hip); allocates nothing.
 constexpr uint64_t kPrScratchBytes = 64ull << 20;

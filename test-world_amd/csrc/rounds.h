@@ -81,6 +81,28 @@ __device__ __forceinline__ uint32_t defer_limit(const RoundCtr *in, uint32_t win
 // the smallest weight. Valid when in->fmin_inv != 0.
 __device__ __forceinline__ uint32_t pull_bound(const RoundCtr *in, uint32_t wmin) { return add_sat(~in->fmin_inv, wmin); }
 
+// Whether round `in` of an SSSP query sweeps: relaxes every edge through the
+// binned layout (pagerank_bins.hip) instead of pushing or pulling. It does
+// when its frontier is held as flags, the pre-pass counted the in-edges of
+// the vertices that can still improve (ucount), and both the push (the
+// frontier's out-edges less those of its deferred vertices) and, in a round
+// whose pull kernel runs (factor > 0), the pull (those in-edges x factor)
+// would scan more than min_edges edges (0: the round never sweeps). The push,
+// pull and sweep kernels of a round all decide with this function.
+__device__ __forceinline__ bool round_sweeps(const RoundCtr *in, uint32_t found, uint64_t edges, uint32_t qcap,
+                                             float factor, uint64_t min_edges) {
+    if (!min_edges || !found || edges <= qcap || in->mode != kModeFlags || !in->ucount) return false;
+    uint64_t u = 0, d = 0;
+#pragma unroll
+    for (int i = 0; i < kShards; i++) {
+        u += in->uedges_sh[i];
+        d += in->dedges_sh[i];
+    }
+    const uint64_t push = edges - min(d, edges);
+    if (push <= min_edges) return false;
+    return !(factor > 0) || (double)u * factor > (double)min_edges;
+}
+
 // Per-thread tallies, summed per workgroup at the end of a round.
 struct Acc {
     uint32_t found = 0;
@@ -516,6 +538,8 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_
     g->frontiers.clear();
     g->pulled_rounds = 0;
     g->pulled_mask = 0;
+    g->swept_rounds = 0;
+    g->swept_mask = 0;
     for (;;) {
         for (uint32_t k = 0; k < batch; k++, r++) {
             int rc = launch(r);
@@ -544,6 +568,8 @@ int drive_rounds(ptg_graph *g, Launch launch, Idle idle, uint32_t &hint, uint32_
             g->frontiers.push_back(s.found);
             g->pulled_rounds += c.pulled;
             if (c.pulled && checked < 64) g->pulled_mask |= 1ull << checked;
+            g->swept_rounds += c.swept;
+            if (c.swept && checked < 64) g->swept_mask |= 1ull << checked;
             rounds++;
         }
         if (done) break;

@@ -241,8 +241,8 @@ static int create_impl(uint32_t V, uint64_t E, const uint32_t *ptr, const uint32
     if (c.small_degree == 0 || c.large_degree < c.small_degree || c.large_degree < 64)
         return fail(PTG_ERR_INVALID_ARG, "graph_create: need 0 < small_degree <= large_degree, large_degree >= 64");
     if (c.sssp_sentinel == 0) c.sssp_sentinel = 0xFFFFFFFEu;
-    if (c.sssp_pull != c.sssp_pull || c.batch_pull != c.batch_pull)
-        return fail(PTG_ERR_INVALID_ARG, "graph_create: NaN sssp_pull or batch_pull");
+    if (c.sssp_pull != c.sssp_pull || c.batch_pull != c.batch_pull || c.sssp_sweep != c.sssp_sweep)
+        return fail(PTG_ERR_INVALID_ARG, "graph_create: NaN sssp_pull, batch_pull or sssp_sweep");
     if (c.pr_block && c.pr_block != 256 && c.pr_block != 512 && c.pr_block != 1024)
         return fail(PTG_ERR_INVALID_ARG, "graph_create: pr_block is 0, 256, 512 or 1024");
     if (c.pr_layout > PTG_PR_BINNED) return fail(PTG_ERR_INVALID_ARG, "graph_create: pr_layout is 0, 1 or 2");
@@ -585,14 +585,15 @@ int ptg_graph_set_config(ptg_graph *g, const ptg_config *c) {
         return fail(PTG_ERR_INVALID_ARG, "only tuning fields can change after graph creation");
     if (!(c->td_to_bu >= 0) || !(c->bu_to_td >= 0) || !(c->batch_td_to_bu >= 0))
         return fail(PTG_ERR_INVALID_ARG, "negative or NaN switch threshold");
-    if (c->sssp_pull != c->sssp_pull || c->batch_pull != c->batch_pull)
-        return fail(PTG_ERR_INVALID_ARG, "NaN sssp_pull or batch_pull");
+    if (c->sssp_pull != c->sssp_pull || c->batch_pull != c->batch_pull || c->sssp_sweep != c->sssp_sweep)
+        return fail(PTG_ERR_INVALID_ARG, "NaN sssp_pull, batch_pull or sssp_sweep");
     if (c->pr_block && c->pr_block != 256 && c->pr_block != 512 && c->pr_block != 1024)
         return fail(PTG_ERR_INVALID_ARG, "pr_block is 0, 256, 512 or 1024");
     if (c->pr_layout > PTG_PR_BINNED) return fail(PTG_ERR_INVALID_ARG, "pr_layout is 0, 1 or 2");
     const bool replan = g->streamed && c->chunk_bytes != o.chunk_bytes;
     g->cfg = *c;
-    g->pull_hist[0] = g->pull_hist[1] = ~0ull;  // SSSP pull rounds are looked for again
+    g->pull_hist[0] = g->pull_hist[1] = ~0ull;  // SSSP pull and sweep rounds are looked for again
+    g->sweep_hist[0] = g->sweep_hist[1] = ~0ull;
     g->pull_queries = 0;
     if (replan) {
         // a streamed graph is cut into chunks of the new size; on failure

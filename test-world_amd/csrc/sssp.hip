@@ -36,6 +36,11 @@ constexpr uint32_t kUnbounded = 0xFFFFFFFFu;
 // (measured best of 32..1024 on R-MAT scale 22 with weights in [1, 64]:
 // 2.59 ms per query against 2.67 at 128 and 3.82 without deferral).
 constexpr uint32_t kDefaultDeferDegree = 64;
+// Default sweep threshold (sssp_sweep 0): a round sweeps when its push and
+// its pull would each scan more than this share of the edges. A sweep reads
+// every edge's 16-bit entry and weight byte and every pair's bin once, which
+// costs about what pushing or pulling a quarter of the edges does.
+constexpr double kDefaultSweepShare = 0.25;
 
 // One relaxation. A candidate (new value below the target's value as read)
 // issues an atomicMin whose result is not waited for, and counts as improved:
@@ -172,6 +177,7 @@ struct PullParams {
     const uint32_t *long_block;            // entry holding chunk b * kPullBlock, per block b
     uint32_t n_long, long_chunks;
     uint32_t tile_deg;                     // lists up to this long are walked in their tile
+    uint64_t sweep;                        // as RelaxParams::sweep for this round
 };
 
 // Pull rounds: lists of up to tile_deg in-edges are walked by the wave that
@@ -387,6 +393,8 @@ struct RelaxParams {
     uint32_t window;
     float pull;          // > 0: a pull kernel follows each round, which pulls when
                          // open in-edges * pull < the push round's out-edges
+    uint64_t sweep;      // > 0: sweep kernels follow the round, which sweep when
+                         // round_sweeps(..., pull, sweep) (rounds.h)
 };
 
 // Heavy-edge round: every vertex whose final value lies in the bucket
@@ -457,8 +465,12 @@ __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxPara
         const uint32_t lim = p.defer_deg != 0xFFFFFFFFu ? defer_limit(in, p.window) : 0xFFFFFFFFu;
         Relax<kWeighted, kWeighted && !kBuckets> op{e.off,   e.col,   p.wt, p.cw,  p.val,       e.fout,
                                                      e.flout, p.far,   thr,  p.defer_deg, lim};
-        // a pulling round is expanded by k_pull_round, launched behind this one
-        if (!(kWeighted && !kBuckets && round_pulls(in, s, e.qcap, p.pull))) expand_frontier(e, in, op, em, acc);
+        // a pulling round is expanded by k_pull_round, a sweeping one by the
+        // sweep kernels, both launched behind this one
+        const bool other = kWeighted && !kBuckets &&
+                           (round_sweeps(in, s.found, s.edges, e.qcap, p.pull, p.sweep) ||
+                            round_pulls(in, s, e.qcap, p.pull));
+        if (!other) expand_frontier(e, in, op, em, acc);
         if (lim != 0xFFFFFFFFu) requeue_deferred(e, in, op, em, acc);
     } else if constexpr (kBuckets) {
         if (heavy)
@@ -478,7 +490,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_relax_round(RoundEnv e, RelaxPara
 __global__ __launch_bounds__(kBlock, 8) void k_pull_round(RoundEnv e, PullParams q) {
     RoundCtr *in = e.ring + e.r % kRing;
     const InSummary s = summarize(in);
-    if (!round_pulls(in, s, e.qcap, q.pull)) return;
+    if (!round_pulls(in, s, e.qcap, q.pull) || round_sweeps(in, s.found, s.edges, e.qcap, q.pull, q.sweep)) return;
     RoundCtr *out = e.ring + (e.r + 1) % kRing;
     if (blockIdx.x == 0 && threadIdx.x == 0) in->pulled = 1u;
     const Emit em = make_emit(e, out, kModeFlags);
@@ -501,12 +513,24 @@ static uint64_t pull_window(ptg_graph *g) {
     return h | (h << 1) | (h >> 1);
 }
 
+// Rounds that launch the sweep kernels: those within one of a round that
+// swept in either of the last two queries, and every round of the queries
+// whose pull window holds every round.
+static uint64_t sweep_window(const ptg_graph *g, bool all) {
+    if (all) return ~0ull;
+    const uint64_t h = g->sweep_hist[0] | g->sweep_hist[1];
+    return h | (h << 1) | (h >> 1);
+}
+
 template <bool kWeighted, class Finish>
 static int run_relax_rounds(ptg_graph *g, RelaxParams p, const PullParams &q, const uint32_t *off,
                             const uint32_t *col, LaunchTimer &timer, uint32_t *iters, uint64_t *fsum, Finish finish,
                             bool *finished) {
+    const bool look_all = g->pull_queries % 16 == 0;
     const uint64_t window = p.pull > 0 ? pull_window(g) : 0ull;
     const float factor = p.pull;
+    const uint64_t sweep_edges = p.sweep;
+    const uint64_t swindow = sweep_edges ? sweep_window(g, look_all) : 0ull;
     int rc = drive_rounds(
         g,
         [&](uint32_t r) -> int {
@@ -515,7 +539,8 @@ static int run_relax_rounds(ptg_graph *g, RelaxParams p, const PullParams &q, co
             env.defer_deg = p.defer_deg;
             env.window = p.window;
             p.pull = factor > 0 && (r >= 64 || ((window >> r) & 1)) ? factor : 0.0f;
-            env.pull = p.pull > 0 ? 1u : 0u;
+            p.sweep = sweep_edges && (r >= 64 || ((swindow >> r) & 1)) ? sweep_edges : 0ull;
+            env.pull = p.pull > 0 || p.sweep ? 1u : 0u;
             env.pull_wmin = q.wmin;
             env.pull_gate = (uint32_t)std::min<uint64_t>(g->E / kPullGateShare, 0xFFFFFFFFu);
             int rc = launch_prepare(g, env);
@@ -524,15 +549,25 @@ static int run_relax_rounds(ptg_graph *g, RelaxParams p, const PullParams &q, co
                 timer.launch(k_relax_round<kWeighted, true>, dim3(g->grid), dim3(kBlock), env, p);
             else
                 timer.launch(k_relax_round<kWeighted, false>, dim3(g->grid), dim3(kBlock), env, p);
-            if (p.pull > 0) timer.launch_rider(k_pull_round, dim3(g->grid), dim3(kBlock), env, q);
+            if (p.pull > 0) {
+                PullParams qr = q;
+                qr.sweep = p.sweep;
+                timer.launch_rider(k_pull_round, dim3(g->grid), dim3(kBlock), env, qr);
+            }
             PTG_HIP(hipGetLastError());
+            if (p.sweep) return sssp_sweep_launch(g, env, p.val, q.sent, p.pull, p.sweep, timer);
             return PTG_OK;
         },
         [](const RoundCtr &c) { return c.any == 0 && c.far_any == 0 && c.need_heavy == 0; },
         g->last_rounds[kWeighted ? 1 : 2], iters, fsum, nullptr, finish, finished);
     if (!rc && factor > 0) {
+        // a round that swept stays a candidate for pulls too
         g->pull_hist[1] = g->pull_hist[0];
-        g->pull_hist[0] = g->pulled_mask;
+        g->pull_hist[0] = g->pulled_mask | g->swept_mask;
+    }
+    if (!rc && sweep_edges) {
+        g->sweep_hist[1] = g->sweep_hist[0];
+        g->sweep_hist[0] = g->swept_mask;
     }
     return rc;
 }
@@ -672,6 +707,7 @@ uint64_t sssp_split_need(const ptg_graph *g) {
     if (pull_wanted(g, delta != kUnbounded) && g->wsymmetric != 1 && !g->d_cwin)
         need += 8ull * std::max<uint64_t>(g->E, 1) + 4ull * g->V + 4;
     if (pull_wanted(g, delta != kUnbounded) && !g->long_built) need += 12ull * g->V + 12;
+    if (pull_wanted(g, delta != kUnbounded)) need += sssp_sweep_need(g);
     if (delta != kUnbounded && !(g->split_delta == delta && g->light.off))
         need += 8ull * g->E + 8ull * (g->V + 1) + 4ull * g->V + (64u << 10);
     return need;
@@ -906,6 +942,14 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
             cwin = wsym ? g->d_cw : g->d_cwin;
         }
     }
+    // sweep rounds over the binned layout where pulls are possible
+    uint64_t sweep_edges = 0;
+    if (cwin && g->cfg.sssp_sweep >= 0) {
+        bool ok = false;
+        if ((rc = sssp_sweep_prepare(g, &ok))) return rc;
+        const double share = g->cfg.sssp_sweep > 0 ? g->cfg.sssp_sweep : kDefaultSweepShare;
+        if (ok) sweep_edges = std::max<uint64_t>(1, (uint64_t)((double)g->E * share));
+    }
     const uint32_t defer_deg = g->cfg.sssp_defer_degree ? g->cfg.sssp_defer_degree : kDefaultDeferDegree;
     const uint32_t window = g->cfg.sssp_defer_window ? g->cfg.sssp_defer_window : g->auto_window;
     const uint32_t *off = split ? g->light.off : g->out.off;
@@ -941,7 +985,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
     RelaxParams p{dist, split ? g->d_wl : g->d_w, split ? nullptr : g->d_cw, g->d_far, delta, split ? 1u : 0u,
                   split ? g->heavy.off : nullptr, split ? g->heavy.col : nullptr, split ? g->d_wh : nullptr,
                   split ? g->heavy.bins.q.with(g->heavy.bins.d_ctr) : Frontier{}, defer_deg, window,
-                  cwin ? (g->cfg.sssp_pull > 0 ? g->cfg.sssp_pull : 1.0f) : 0.0f};
+                  cwin ? (g->cfg.sssp_pull > 0 ? g->cfg.sssp_pull : 1.0f) : 0.0f, sweep_edges};
     const PullParams q{cwin,          dist,           p.pull,          g->wmin,  g->cfg.sssp_sentinel,
                        g->d_long_ids, g->d_long_base, g->d_long_block, g->n_long, g->long_chunks,
                        g->pull_tile_deg};
@@ -959,6 +1003,7 @@ int run_sssp(ptg_graph *g, uint32_t src, uint32_t *dist, ptg_stats *st) {
         st->iterations = iters;
         st->frontier_sum = fsum;
         st->pull_rounds = g->pulled_rounds;
+        st->sweep_rounds = g->swept_rounds;
         if ((rc = timer.collect(st))) return rc;
         reach_stats_read(g, &st->reached, &st->edges_traversed);
     }

@@ -83,6 +83,7 @@ class Config(ctypes.Structure):
         ("batch_pull", ctypes.c_float),
         ("pr_block", ctypes.c_uint32),
         ("pr_layout", ctypes.c_uint32),
+        ("sssp_sweep", ctypes.c_float),
     ]
 
 
@@ -100,6 +101,7 @@ class Stats(ctypes.Structure):
         ("chunk_loads", ctypes.c_uint32),
         ("chunk_hits", ctypes.c_uint32),
         ("pull_rounds", ctypes.c_uint32),
+        ("sweep_rounds", ctypes.c_uint32),
     ]
 
     def as_dict(self):

@@ -41,10 +41,13 @@ After the timed steps (none of this is in `value`):
            700/s (seed 4, near the plain path's saturation) under a 24 GiB
            query-state budget (the graphs' batched state alone would take
            about 27 GB, so the budget forces evictions), each graph autotuned
-           first for single queries and batches: the plain per-query path
-           (one at a time) vs the server (continuous batching); p50/p99,
-           mean batch and evictions at that load, and queries/s with every
-           request queued at once; checksums must match.
+           first for single queries and batches (PageRank by the in-list pull
+           and SSSP without sweep rounds, whose binned layout would crowd the
+           budget): the plain per-query path (one at a time) vs the server
+           (continuous batching); p50/p99, mean batch and evictions at that
+           load, at the same arrivals compressed to 1600/s (above the plain
+           path's saturation), and with every request queued at once; every
+           request's checksum must be the same in every phase of both.
 """
 import argparse
 import json

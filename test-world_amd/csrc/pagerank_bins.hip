@@ -8,12 +8,11 @@
 //
 //   partitions  destinations in ranges of kPW = 16384 vertices; a workgroup
 //               keeps the 16384 sums of one range in LDS (128 KB of Q1.63);
-//   pairs       one per (source u, partition p) with an edge u -> v, v in p.
-//               bins[j] holds r(u)/outdeg(u) of pair j, written once per
-//               iteration when u is finished. Pairs are ordered by
-//               partition, then source. In the default degree order most of
-//               a vertex's edges go to the hubs, which share partition 0, so
-//               there are far fewer pairs than edges (0.16 E at R-MAT scale 22);
+//   pairs       one per (source u, partition p) with an edge u -> v, v in p,
+//               ordered by partition, then source; bsrc[j] is pair j's
+//               source. In the default degree order most of a vertex's edges
+//               go to the hubs, which share partition 0, so there are far
+//               fewer pairs than edges (0.16 E at R-MAT scale 22);
 //   entries     one 16-bit word per edge, in pair order and, within a pair,
 //               by destination: the destination's offset in its partition
 //               (14 bits) and, in bit 15, whether the edge starts a new pair.
@@ -29,13 +28,17 @@
 //               (the 16 waves split the rows).
 //
 // Per iteration, k_prb_acc streams the entries (2 bytes per edge) and the
-// bins (4 bytes per pair, read in order) and adds each edge's term into LDS
-// with a 64-bit LDS atomic; k_prb_fin adds up every vertex's rows and
-// finishes it: its rank, its share of the dangling mass and the next
-// iteration's bins, written through pair_pos (for each source, the bin index
-// of each of its pairs). Workgroup 0 of k_prb_acc sums the previous
-// iteration's dangling partials into the base value k_prb_fin uses. Kernel
-// boundaries order everything.
+// pairs' sources (4 bytes per pair, read in order), reads each pair's
+// contribution r(u)/outdeg(u) (within a partition the sources ascend, so a
+// wave's reads share lines where the sources are dense) and adds each
+// edge's term into LDS with a 64-bit LDS atomic; k_prb_fin adds up every
+// vertex's rows and finishes it: its rank, its share of the dangling mass
+// and its contribution for the next iteration (one coalesced store per
+// vertex). Workgroup 0 of k_prb_acc sums the previous iteration's dangling
+// partials into the base value k_prb_fin uses. Kernel boundaries order
+// everything. (Writing each pair's contribution to a bin in pair order
+// instead, so that k_prb_acc reads the bins as a stream, took 20 M scattered
+// stores per iteration at R-MAT scale 22 and was slower.)
 //
 // The layout is built once per graph on the host from the device out-lists
 // (the edges u -> v themselves, so a directed graph needs no transpose) and
@@ -43,13 +46,12 @@
 // is a plain copy.
 //
 // SSSP sweep rounds use the same layout with one more array, a weight byte
-// per entry (graphs whose weights are at most 255) and the source of every
-// pair in bin order. A sweep relaxes every edge u -> v with the value u had
-// when the round began: k_sw_acc reads each entry's source value through its
-// pair's source (in a partition the pairs' sources ascend, so the reads of a
-// wave share lines where the sources are dense), takes per chunk the
-// minimum of bin + weight over the entries of each destination in LDS (a
-// 32-bit LDS atomicMin) and writes the chunk's row of minima, and k_sw_fin
+// per entry (graphs whose weights are at most 255). A sweep relaxes every
+// edge u -> v with the value u had when the round began: k_sw_acc reads each
+// entry's source value through its pair's source, as k_prb_acc reads
+// contributions, takes per chunk the minimum of value + weight over the
+// entries of each destination in LDS (a 32-bit LDS atomicMin) and writes the
+// chunk's row of minima, and k_sw_fin
 // takes each vertex's minimum over its partition's rows; a vertex whose
 // minimum is below its value takes it and joins the next frontier, as a
 // pushed vertex does. Relaxing an edge out of a vertex outside the frontier
@@ -102,9 +104,9 @@ struct PrbUnit {
 struct PrBinsHost {
     std::vector<uint16_t> ent;
     std::vector<uint8_t> went;  // SSSP sweeps: the weight of every entry (0 for nulls); empty when not built
-    std::vector<uint32_t> bsrc; // SSSP sweeps: the source of every pair, in bin order
+    std::vector<uint32_t> bsrc; // the source of every pair, in pair order
     bool has_w = false;
-    std::vector<uint32_t> pos, pb, seg_cur;
+    std::vector<uint32_t> seg_cur;
     std::vector<PrbChunk> chunks;
     std::vector<PrbUnit> units;
     uint32_t n_rows = 0;
@@ -115,14 +117,15 @@ struct PrBins {
     PrBinsHost h;
     bool built = false;
     bool usable = false;  // auto layout: false when pairs would exceed half the edges
+    bool sweep_refused = false;  // SSSP sweeps: the layout would not pay (pairs above half the edges)
     // device copy
     uint16_t *ent = nullptr;
     uint8_t *went = nullptr;
     uint32_t *bsrc = nullptr;
-    uint32_t *pos = nullptr, *pb = nullptr, *seg_cur = nullptr;
+    uint32_t *seg_cur = nullptr;
     PrbChunk *chunks = nullptr;
     PrbUnit *units = nullptr;
-    float *bins[2] = {};
+    float *con[2] = {};  // contribution r(v)/outdeg(v) of every vertex, two iterations in turn
     unsigned long long *rows = nullptr;
     unsigned long long *part = nullptr;  // 2 iterations x units, then the base value
     bool on_device = false;
@@ -134,10 +137,9 @@ struct PrbArgs {
     const PrbChunk *chunks;
     const PrbUnit *units;
     const uint32_t *ooff;  // out offsets (outdeg)
-    const uint32_t *pb;    // V + 1: first pair of each source
-    const uint32_t *pos;   // bin index of every pair, by source
-    const float *bin_in;
-    float *bin_out;
+    const uint32_t *bsrc;  // the source of every pair
+    const float *con_in;   // contributions of this iteration
+    float *con_out;        // ... of the next
     float *rank_out;       // last iteration only
     unsigned long long *rows;
     const unsigned long long *part_in;  // previous iteration's dangling partials (one per unit)
@@ -147,7 +149,7 @@ struct PrbArgs {
     uint32_t n_vert;
     uint32_t last_pair;    // pairs - 1
     bool first;            // iteration 0: the dangling mass is d0
-    bool init;             // k_prb_fin writes the bins of r0 = 1/V only
+    bool init;             // k_prb_fin writes the contributions of r0 = 1/V only
     double d0;
     float damping, inv_v;
 };
@@ -167,36 +169,6 @@ __device__ __forceinline__ void prb_base(const PrbArgs &a, unsigned long long *l
         dang = fx63_to_f64(s);
     }
     if (threadIdx.x == 0) *a.base = pr_base_value(a.damping, a.inv_v, dang);
-}
-
-// Writes the contributions s_c[i] of the nv vertices whose first pairs are
-// s_pb[0..nv] to the bins of all their pairs; the workgroup's threads walk
-// the pairs in order (pair_pos reads coalesced), four at a time.
-__device__ __forceinline__ void prb_scatter(const PrbArgs &a, uint32_t nv, const uint32_t *s_pb, const float *s_c) {
-    const uint32_t k0 = s_pb[0], k1 = s_pb[nv];
-    for (uint32_t k = k0 + threadIdx.x; k < k1; k += 4 * kAccBlock) {
-        uint32_t at[4];
-        float c[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint32_t kk = k + i * kAccBlock;
-            if (kk < k1) {
-                uint32_t lo = 0, hi = nv;  // s_pb[lo] <= kk < s_pb[hi]
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_pb[mid] <= kk)
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                at[i] = a.pos[kk];
-                c[i] = s_c[lo];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (k + i * kAccBlock < k1) a.bin_out[at[i]] = c[i];
-    }
 }
 
 // One workgroup per chunk: adds every entry's term into LDS sums of its
@@ -226,6 +198,7 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_acc(PrbArgs a) {
             uint64_t nxt[kGroups];
 #pragma unroll
             for (int q = 0; q < kGroups; q++) nxt[q] = load4(g + 256ull * (kGroups + q) + 4ull * lane);
+            uint32_t src[kGroups][4];
             float x[kGroups][4];
 #pragma unroll
             for (int q = 0; q < kGroups; q++) {
@@ -241,9 +214,14 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_acc(PrbArgs a) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     pr += (uint32_t)(w4 >> (16 * i + 15)) & 1u;
-                    x[q][i] = a.bin_in[min(pr, a.last_pair)];
+                    src[q][i] = a.bsrc[min(pr, a.last_pair)];
                 }
             }
+            // every pair's contribution read before any is used
+#pragma unroll
+            for (int q = 0; q < kGroups; q++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) x[q][i] = a.con_in[src[q][i]];
 #pragma unroll
             for (int q = 0; q < kGroups; q++)
 #pragma unroll
@@ -262,13 +240,11 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_acc(PrbArgs a) {
     for (uint32_t i = threadIdx.x; i < kPW; i += kAccBlock) row[i] = s_sum[i];
 }
 
-// One workgroup per unit: adds up each vertex's rows, finishes it (rank,
-// dangling share, contribution) and writes its bins. With `init`, writes the
-// bins of r0 = 1/V instead.
+// One workgroup per unit: adds up each vertex's rows and finishes it (rank,
+// dangling share, contribution for the next iteration). With `init`, writes
+// the contributions of r0 = 1/V instead.
 __global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
     __shared__ unsigned long long s_part[kAccWaves][kTile];
-    __shared__ uint32_t s_pb[kTile + 1];
-    __shared__ float s_c[kTile];
     __shared__ unsigned long long s_red[kAccWaves];
     const PrbUnit u = a.units[blockIdx.x];
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
@@ -287,11 +263,10 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
         }
     }
     unsigned long long dang = 0;
-    float c = 0.0f;
-    uint32_t kb = 0, ke = 0;
     if (threadIdx.x < u.nv) {
         const uint32_t v = u.v0 + threadIdx.x;
         const uint32_t od = a.ooff[v + 1] - a.ooff[v];
+        float c;
         if (a.init) {
             c = od ? a.inv_v / (float)od : 0.0f;
         } else {
@@ -300,29 +275,7 @@ __global__ __launch_bounds__(kAccBlock) void k_prb_fin(PrbArgs a) {
             if (!od) dang = fx63(r);
             c = od ? r / (float)od : 0.0f;
         }
-        kb = a.pb[v];
-        ke = a.pb[v + 1];
-    }
-    if (u.nrows > kRowsDirect) {
-        // hub units: many pairs per vertex, spread over the workgroup
-        if (threadIdx.x < u.nv) {
-            s_c[threadIdx.x] = c;
-            s_pb[threadIdx.x] = kb;
-        }
-        if (threadIdx.x == 0) s_pb[u.nv] = a.pb[u.v0 + u.nv];
-        __syncthreads();
-        prb_scatter(a, u.nv, s_pb, s_c);
-    } else {
-        // a thread per vertex, four pairs at a time
-        for (uint32_t k = kb; k < ke; k += 4) {
-            uint32_t at[4];
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++)
-                if (k + i < ke) at[i] = a.pos[k + i];
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++)
-                if (k + i < ke) a.bin_out[at[i]] = c;
-        }
+        a.con_out[v] = c;
     }
     if (a.init) return;
     dang = wave_sum_u64(dang);
@@ -365,24 +318,21 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
         vb[t] = (uint32_t)(std::upper_bound(off, off + V, (uint32_t)target) - off);
         vb[t] = std::max(vb[t], vb[t - 1]);
     }
-    // pass 1: pairs and entries per (thread, partition), pairs per source
+    // pass 1: pairs and entries per (thread, partition)
     std::vector<std::vector<uint64_t>> cp(T, std::vector<uint64_t>(P, 0)), ce(T, std::vector<uint64_t>(P, 0));
-    std::vector<uint32_t> np(V);
     run_ranges(vb, [&](int t, uint32_t b, uint32_t e) {
         auto &mp = cp[t];
         auto &me = ce[t];
         for (uint32_t u = b; u < e; u++) {
-            uint32_t n = 0, prev = 0xFFFFFFFFu;
+            uint32_t prev = 0xFFFFFFFFu;
             for (uint32_t k = off[u]; k < off[u + 1]; k++) {
                 const uint32_t p = col[k] >> kPWBits;
                 me[p]++;
                 if (p != prev) {
                     mp[p]++;
-                    n++;
                     prev = p;
                 }
             }
-            np[u] = n;
         }
     });
     uint64_t pairs = 0;
@@ -412,26 +362,21 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
             q += cp[t][p];
         }
     }
-    out->pb.assign((size_t)V + 1, 0);
-    for (uint32_t u = 0; u < V; u++) out->pb[u + 1] = out->pb[u] + np[u];
     out->n_pairs = pairs;
-    out->pos.assign(pairs ? pairs : 1, 0);
+    out->bsrc.assign(pairs ? pairs : 1, 0);
     out->ent.assign(ent_off[P] ? ent_off[P] : 4, kEntNull);
     out->has_w = w != nullptr;
-    if (w) {
+    if (w)
         out->went.assign(out->ent.size(), 0);
-        out->bsrc.assign(out->pos.size(), 0);
-    } else {
+    else
         std::vector<uint8_t>().swap(out->went);
-        std::vector<uint32_t>().swap(out->bsrc);
-    }
     out->seg_cur.assign(seg_off[P] ? seg_off[P] : 1, 0);
-    // pass 2: entries, pair positions, segment starts
+    // pass 2: entries, the pairs' sources, segment starts
     run_ranges(vb, [&](int t, uint32_t b, uint32_t e) {
         auto &ecur = ec[t];
         auto &pcur = pc[t];
         for (uint32_t u = b; u < e; u++) {
-            uint32_t prev = 0xFFFFFFFFu, kp = out->pb[u];
+            uint32_t prev = 0xFFFFFFFFu;
             uint32_t pp = 0;
             for (uint32_t k = off[u]; k < off[u + 1]; k++) {
                 const uint32_t v = col[k];
@@ -439,8 +384,7 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
                 const bool first = p != prev;
                 if (first) {
                     pp = (uint32_t)pcur[p]++;
-                    out->pos[kp++] = pp;
-                    if (w) out->bsrc[pp] = u;
+                    out->bsrc[pp] = u;
                     prev = p;
                 }
                 const uint64_t x = ecur[p]++;
@@ -487,33 +431,28 @@ static bool build_layout(uint32_t V, uint64_t E, const uint32_t *off, const uint
 static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
 
 static uint64_t device_bytes_of(const PrBinsHost &h, uint32_t V) {
-    return al256(2 * h.ent.size()) + al256(h.went.size()) + al256(4 * h.bsrc.size()) + al256(4 * h.pos.size()) +
-           al256(4ull * ((uint64_t)V + 1)) +
-           al256(4 * h.seg_cur.size()) + al256(sizeof(PrbChunk) * h.chunks.size()) +
-           al256(sizeof(PrbUnit) * h.units.size()) + 2 * al256(4 * h.pos.size()) +
-           al256(8ull * kPW * h.n_rows) + al256(8 * (2 * h.units.size() + 1));
+    return al256(2 * h.ent.size()) + al256(h.went.size()) + al256(4 * h.bsrc.size()) + al256(4 * h.seg_cur.size()) +
+           al256(sizeof(PrbChunk) * h.chunks.size()) + al256(sizeof(PrbUnit) * h.units.size()) +
+           2 * al256(4ull * V) + al256(8ull * kPW * h.n_rows) + al256(8 * (2 * h.units.size() + 1));
 }
 
 static void free_device(ptg_graph *g, PrBins *b) {
     gfree(g, b->ent);
     gfree(g, b->went);
     gfree(g, b->bsrc);
-    gfree(g, b->pos);
-    gfree(g, b->pb);
     gfree(g, b->seg_cur);
     gfree(g, b->chunks);
     gfree(g, b->units);
-    gfree(g, b->bins[0]);
-    gfree(g, b->bins[1]);
+    gfree(g, b->con[0]);
+    gfree(g, b->con[1]);
     gfree(g, b->rows);
     gfree(g, b->part);
     b->ent = nullptr;
     b->went = nullptr;
-    b->bsrc = nullptr;
-    b->pos = b->pb = b->seg_cur = nullptr;
+    b->bsrc = b->seg_cur = nullptr;
     b->chunks = nullptr;
     b->units = nullptr;
-    b->bins[0] = b->bins[1] = nullptr;
+    b->con[0] = b->con[1] = nullptr;
     b->rows = b->part = nullptr;
     b->on_device = false;
 }
@@ -521,28 +460,24 @@ static void free_device(ptg_graph *g, PrBins *b) {
 static int upload(ptg_graph *g, PrBins *b) {
     const PrBinsHost &h = b->h;
     int rc;
-    if ((rc = galloc(g, (void **)&b->ent, 2 * h.ent.size())) || (rc = galloc(g, (void **)&b->pos, 4 * h.pos.size())) ||
-        (rc = galloc(g, (void **)&b->pb, 4 * h.pb.size())) ||
+    if ((rc = galloc(g, (void **)&b->ent, 2 * h.ent.size())) ||
+        (rc = galloc(g, (void **)&b->bsrc, 4 * h.bsrc.size())) ||
         (rc = galloc(g, (void **)&b->seg_cur, 4 * h.seg_cur.size())) ||
         (rc = galloc(g, (void **)&b->chunks, sizeof(PrbChunk) * h.chunks.size())) ||
         (rc = galloc(g, (void **)&b->units, sizeof(PrbUnit) * h.units.size())) ||
-        (rc = galloc(g, (void **)&b->bins[0], 4 * h.pos.size())) ||
-        (rc = galloc(g, (void **)&b->bins[1], 4 * h.pos.size())) ||
+        (rc = galloc(g, (void **)&b->con[0], 4ull * g->V)) || (rc = galloc(g, (void **)&b->con[1], 4ull * g->V)) ||
         (rc = galloc(g, (void **)&b->rows, 8ull * kPW * h.n_rows)) ||
         (rc = galloc(g, (void **)&b->part, 8 * (2 * h.units.size() + 1))) ||
-        (h.has_w && (rc = galloc(g, (void **)&b->went, h.went.size()))) ||
-        (h.has_w && (rc = galloc(g, (void **)&b->bsrc, 4 * h.bsrc.size())))) {
+        (h.has_w && (rc = galloc(g, (void **)&b->went, h.went.size())))) {
         free_device(g, b);
         return rc;
     }
     if ((rc = copy_sync(g, b->ent, h.ent.data(), 2 * h.ent.size(), hipMemcpyHostToDevice)) ||
-        (rc = copy_sync(g, b->pos, h.pos.data(), 4 * h.pos.size(), hipMemcpyHostToDevice)) ||
-        (rc = copy_sync(g, b->pb, h.pb.data(), 4 * h.pb.size(), hipMemcpyHostToDevice)) ||
+        (rc = copy_sync(g, b->bsrc, h.bsrc.data(), 4 * h.bsrc.size(), hipMemcpyHostToDevice)) ||
         (rc = copy_sync(g, b->seg_cur, h.seg_cur.data(), 4 * h.seg_cur.size(), hipMemcpyHostToDevice)) ||
         (rc = copy_sync(g, b->chunks, h.chunks.data(), sizeof(PrbChunk) * h.chunks.size(), hipMemcpyHostToDevice)) ||
         (rc = copy_sync(g, b->units, h.units.data(), sizeof(PrbUnit) * h.units.size(), hipMemcpyHostToDevice)) ||
-        (h.has_w && (rc = copy_sync(g, b->went, h.went.data(), h.went.size(), hipMemcpyHostToDevice))) ||
-        (h.has_w && (rc = copy_sync(g, b->bsrc, h.bsrc.data(), 4 * h.bsrc.size(), hipMemcpyHostToDevice)))) {
+        (h.has_w && (rc = copy_sync(g, b->went, h.went.data(), h.went.size(), hipMemcpyHostToDevice)))) {
         free_device(g, b);
         return rc;
     }
@@ -595,9 +530,12 @@ bool pr_bins_wanted(const ptg_graph *g) { return g->cfg.pr_layout != PTG_PR_PULL
 uint64_t pr_bins_need(const ptg_graph *g) {
     const PrBins *b = g->prb;
     if (b && b->built) return b->usable && !b->on_device ? device_bytes_of(b->h, g->V) : 0;
-    // not built yet: entries, and pairs at most E / 2 (12 bytes each)
-    return al256(2 * (g->E + 4ull * ((g->V + kPW - 1) / kPW))) + 3 * al256(4 * (g->E / 2 + 1)) +
-           al256(4ull * ((uint64_t)g->V + 1)) + al256(8ull * kPW * 64) + (1u << 20);
+    // not built yet: entries, pairs at most E / 2 (a source each), two
+    // contribution arrays, and a row per partition and per extra chunk
+    const uint64_t P = (g->V + kPW - 1) / kPW;
+    const uint64_t rows = 2 * P + kChunksPerCu * (uint64_t)std::max(1, g->num_cus);
+    return al256(2 * (g->E + 4 * P)) + al256(4 * (g->E / 2 + 1)) + 2 * al256(4ull * g->V) +
+           al256(8ull * kPW * rows) + al256(4 * (g->E / kSeg + P + 1)) + (1u << 20);
 }
 
 void release_pr_bins(ptg_graph *g) {
@@ -638,8 +576,7 @@ int run_pagerank_bins(ptg_graph *g, float damping, int iters, float *ranks, ptg_
         a.chunks = b->chunks;
         a.units = b->units;
         a.ooff = g->out.off;
-        a.pb = b->pb;
-        a.pos = b->pos;
+        a.bsrc = b->bsrc;
         a.rows = b->rows;
         a.base = reinterpret_cast<float *>(b->part + 2ull * nu);
         a.n_units = nu;
@@ -649,13 +586,13 @@ int run_pagerank_bins(ptg_graph *g, float damping, int iters, float *ranks, ptg_
         a.damping = damping;
         a.inv_v = r0;
         a.init = true;
-        a.bin_out = b->bins[0];
+        a.con_out = b->con[0];
         k_prb_fin<<<nu, kAccBlock, 0, g->stream>>>(a);
         PTG_HIP(hipGetLastError());
         a.init = false;
         for (int t = 0; t < iters; t++) {
-            a.bin_in = b->bins[t & 1];
-            a.bin_out = b->bins[(t + 1) & 1];
+            a.con_in = b->con[t & 1];
+            a.con_out = b->con[(t + 1) & 1];
             a.rank_out = t == iters - 1 ? ranks : nullptr;
             a.part_in = b->part + (uint64_t)nu * ((t + 1) & 1);
             a.part_out = b->part + (uint64_t)nu * (t & 1);
@@ -859,15 +796,32 @@ int sssp_sweep_prepare(ptg_graph *g, bool *ok) {
     if (!g->weighted || g->cfg.sssp_sweep < 0 || g->wmax > 255 || g->E == 0) return PTG_OK;
     if (!g->prb) g->prb = new PrBins();
     PrBins *b = g->prb;
+    // refused once, refused for good: the automatic layout's rule (pairs at
+    // most half the edges), whether a forced PageRank layout exists or not
+    if (b->sweep_refused) return PTG_OK;
+    if (b->built && (!b->usable || 2 * b->h.n_pairs > g->E)) {
+        b->sweep_refused = true;
+        return PTG_OK;
+    }
     int rc;
     if (!b->built || !b->h.has_w) {
-        // (re)built with the weight bytes; the entries and pairs come out the same
+        // (re)built with the weight bytes; the entries and pairs come out the
+        // same. A refusal builds nothing and leaves the automatic layout's
+        // answer (not usable), which PageRank reads the same way.
         free_device(g, b);
-        b->built = false;
-        if ((rc = build_from_device(g, true, &b->usable, true))) return rc;
+        bool usable = false;
+        if ((rc = build_from_device(g, true, &usable, true))) return rc;
+        if (!usable) {
+            b->sweep_refused = true;
+            if (!b->built) {
+                b->built = true;
+                b->usable = false;
+            }
+            return PTG_OK;
+        }
         b->built = true;
+        b->usable = true;
     }
-    if (!b->usable) return PTG_OK;
     if (!b->on_device && (rc = upload(g, b))) return rc;
     *ok = true;
     return PTG_OK;
@@ -876,6 +830,7 @@ int sssp_sweep_prepare(ptg_graph *g, bool *ok) {
 uint64_t sssp_sweep_need(const ptg_graph *g) {
     if (!g->weighted || g->cfg.sssp_sweep < 0 || (g->auto_window && g->wmax > 255)) return 0;
     const PrBins *b = g->prb;
+    if (b && b->sweep_refused) return 0;
     if (b && b->built && b->h.has_w) return b->usable && !b->on_device ? device_bytes_of(b->h, g->V) : 0;
     // not built with weights yet: the layout's bound plus a byte per entry
     return pr_bins_need(g) + ((g->E + 4ull * ((g->V + kPW - 1) / kPW) + 255) & ~255ull);

@@ -158,3 +158,37 @@ def test_sweeps_and_binned_pagerank_share_the_layout():
             r, _ = g.pagerank(0.85, 20)
             np.testing.assert_array_equal(r.view(np.uint32), pull_ranks.view(np.uint32))
         assert g.sssp(srcs[0])[1].sweep_rounds > 0
+
+
+def test_sweep_refused_when_pairs_outnumber_half_the_edges():
+    """A uniform random graph spreads each list over many partitions: the
+    (source, partition) pairs exceed half the edges, the layout is refused
+    once (and not rebuilt by later queries), nothing sweeps, and PageRank
+    takes the in-list pull; the distances stay Dijkstra's."""
+    rng = np.random.default_rng(9)
+    V, M = 20 * 16384, 1300000
+    a, b = rng.integers(0, V, M), rng.integers(0, V, M)
+    keep = a != b
+    a, b = np.concatenate([a[keep], b[keep]]), np.concatenate([b[keep], a[keep]])
+    order = np.lexsort((b, a))
+    a, b = a[order], b[order]
+    ptr = np.searchsorted(a, np.arange(V)).astype(np.uint32)
+    col = b.astype(np.uint32)
+    w = np.empty(col.size, np.uint32)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    w[:] = (lo * 2654435761 + hi * 40503) % 64 + 1  # one weight per undirected pair
+    srcs = sources(np.append(ptr, col.size), 2, 36)
+    want = {s: oracle.dijkstra(ptr, col, w, s) for s in srcs}
+    with pg.Graph(ptr, col, w, config=pg.default_config(sssp_sweep=1e-9)) as g:
+        assert g.is_symmetric()
+        for rep in range(2):
+            for s in srcs:
+                say("case refused layout rep %d src %d" % (rep, s))
+                d, st = g.sssp(s)
+                np.testing.assert_array_equal(d, want[s])
+                assert st.sweep_rounds == 0
+        say("case refused layout: PageRank")
+        auto, _ = g.pagerank(0.85, 10)
+        g.tune(pr_layout=pg.PR_PULL)
+        pull, _ = g.pagerank(0.85, 10)
+        np.testing.assert_array_equal(auto.view(np.uint32), pull.view(np.uint32))

@@ -485,6 +485,12 @@ static int upload(ptg_graph *g, PrBins *b) {
     return PTG_OK;
 }
 
+// Whether the layout's device copy fits the graph's device budget (galloc
+// refuses allocations past it).
+static bool fits_budget(const ptg_graph *g, const PrBins *b) {
+    return !g->cfg.device_budget_bytes || g->device_bytes + device_bytes_of(b->h, g->V) <= g->cfg.device_budget_bytes;
+}
+
 // Builds the layout from the device out-lists (with a weight byte per entry
 // when with_w; the weights must be at most 255).
 static int build_from_device(ptg_graph *g, bool require_gain, bool *usable, bool with_w = false) {
@@ -520,6 +526,9 @@ static int ensure_bins(ptg_graph *g, bool *use) {
         if (!forced) return PTG_OK;
         if ((rc = build_from_device(g, false, &b->usable))) return rc;  // refused earlier by the automatic layout
     }
+    // the automatic layout gives way to the in-list pull when the layout does
+    // not fit the graph's device budget
+    if (!b->on_device && !forced && !fits_budget(g, b)) return PTG_OK;
     if (!b->on_device && (rc = upload(g, b))) return rc;
     *use = true;
     return PTG_OK;
@@ -822,6 +831,9 @@ int sssp_sweep_prepare(ptg_graph *g, bool *ok) {
         b->built = true;
         b->usable = true;
     }
+    // sweeps are optional: without room in the device budget the rounds push
+    // and pull
+    if (!b->on_device && !fits_budget(g, b)) return PTG_OK;
     if (!b->on_device && (rc = upload(g, b))) return rc;
     *ok = true;
     return PTG_OK;

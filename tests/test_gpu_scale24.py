@@ -3,8 +3,10 @@ sources per launch on R-MAT scale 24 (V = 16,777,216, about 520 M directed
 edges) and on an Erdos-Renyi graph with the same V and edge factor. Rows 0,
 31 and 63 are compared bit for bit with the oracle (the reference loop for
 BFS, Dijkstra for SSSP) and every row is checked by the shortest-path
-certificate (orc_certify_paths). At this size E * 4 bytes passes 2 GB, where
-32-bit byte offsets would wrap."""
+certificate (orc_certify_paths); the single SSSP queries from the sources
+of those rows (with a sweep round on R-MAT) are compared with the same
+oracle rows. At this size E * 4 bytes passes 2 GB, where 32-bit byte
+offsets would wrap."""
 import os
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -86,3 +88,14 @@ def test_batched_sssp_at_scale24(graph24):
         np.testing.assert_array_equal(rows[k], wk, err_msg="%s row %d" % (name, k))
     for k in range(64):
         assert oracle.certify_paths(ptr, col, w, int(srcs[k]), rows[k], SENT, threads=T), (name, k)
+    # the single-query path from the same sources (on R-MAT it pushes, pulls
+    # and sweeps its densest round; on Erdos-Renyi the binned layout is
+    # refused) against the same oracle rows
+    swept = 0
+    for k, wk in zip(ROWS, want):
+        say("case %s single sssp row %d" % (name, k))
+        d, st = g.sssp(int(srcs[k]))
+        np.testing.assert_array_equal(d, wk, err_msg="%s single query %d" % (name, k))
+        swept += st.sweep_rounds
+    # a source in a small component has no dense round to sweep
+    assert swept > 0 if name == "rmat24" else swept == 0, (name, swept)

@@ -192,3 +192,37 @@ def test_sweep_refused_when_pairs_outnumber_half_the_edges():
         g.tune(pr_layout=pg.PR_PULL)
         pull, _ = g.pagerank(0.85, 10)
         np.testing.assert_array_equal(auto.view(np.uint32), pull.view(np.uint32))
+
+
+def test_layout_gives_way_under_a_device_budget():
+    """The binned layout is optional: under a device budget with no room for
+    it, SSSP runs without sweeps and PageRank (automatic layout) by the
+    in-list pull, with the same results; without the budget the same graph
+    sweeps."""
+    csr = pg.gen_rmat(15, 16, seed=7).add_weights(1, 64, seed=8)
+    ptr, col, w = csr.node_pointers.copy(), csr.cols.copy(), csr.weights.copy()
+    srcs = sources(csr.offsets, 2, 37)
+    want = {s: oracle.dijkstra(ptr, col, w, s) for s in srcs}
+    want_pr = oracle.pagerank_f64(ptr, col, 0.85, 10)
+    # the device bytes of everything the queries need but the binned layout
+    with pg.Graph(ptr, col, w, config=pg.default_config(sssp_sweep=-1.0, pr_layout=pg.PR_PULL)) as g:
+        for s in srcs:
+            g.sssp(s)
+        g.pagerank(0.85, 10)
+        base = g.device_bytes
+    budget = base + (1 << 20)
+    with pg.Graph(ptr, col, w, config=pg.default_config(sssp_sweep=1e-9, device_budget_bytes=budget)) as g:
+        for s in srcs:
+            say("case tight budget sssp src %d" % s)
+            d, st = g.sssp(s)
+            np.testing.assert_array_equal(d, want[s])
+            assert st.sweep_rounds == 0
+        say("case tight budget pagerank")
+        r, _ = g.pagerank(0.85, 10)
+        assert float(np.abs(r.astype(np.float64) - want_pr).sum()) <= 1e-6
+        assert g.device_bytes <= budget
+    with pg.Graph(ptr, col, w, config=pg.default_config(sssp_sweep=1e-9)) as g:
+        say("case no budget sweeps")
+        d, st = g.sssp(srcs[0])
+        np.testing.assert_array_equal(d, want[srcs[0]])
+        assert st.sweep_rounds > 0

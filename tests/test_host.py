@@ -76,8 +76,21 @@ def test_budget_rejected_before_allocation():
     assert e.value.code == pg.ERR_BUDGET
 
 
+@pytest.mark.parametrize("field", ["sssp_pull", "batch_pull", "sssp_sweep"])
+def test_nan_tuning_rejected_before_allocation(field):
+    """A NaN decision factor would make every comparison false and silently
+    change the schedule; graph creation refuses it before any device work."""
+    g = pg.gen_rmat(8).add_weights()
+    cfg = pg.default_config(**{field: float("nan")})
+    with pytest.raises(pg.PtgError) as e:
+        pg.Graph.from_csr(g, config=cfg)
+    assert e.value.code == pg.ERR_INVALID_ARG
+
+
 def test_config_defaults():
     c = pg.default_config()
+    # PageRank layout automatic (binned where it pays), SSSP sweeps automatic
+    assert c.pr_layout == pg.PR_AUTO and c.sssp_sweep == 0.0
     assert c.sssp_sentinel == 0xFFFFFFFE
     assert c.small_degree < c.large_degree
     assert c.chunk_bytes == 256 << 20

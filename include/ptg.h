@@ -96,7 +96,7 @@ typedef struct ptg_config {
                                      vertices of degree > d by degree, then breadth-first from them */
     uint32_t sssp_defer_degree;   /* SSSP: a vertex of degree > this waits in the frontier while its
                                      distance >= the frontier minimum + sssp_defer_window;
-                                     0 = auto (64), 0xFFFFFFFF = never */
+                                     0 = auto (128), 0xFFFFFFFF = never */
     uint32_t sssp_defer_window;   /* 0 = auto (mean weight / 16, at least 1) */
     uint32_t stream_pinned;       /* PTG_FLAG_STREAM: the first stream_pinned chunks stay in their
                                      slots once loaded (at most slots - 2); in the default degree

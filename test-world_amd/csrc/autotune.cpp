@@ -162,10 +162,10 @@ extern "C" int ptg_autotune(ptg_graph *g, const uint32_t *srcs, int n, uint32_t 
         if (!rc) rc = t.knob(kBfs, &c.bu_to_td, std::vector<float>{8, 16, 24, 48, 96, 200});
     }
     if ((what & PTG_TUNE_SSSP) && g->weighted) {
-        // 0 = automatic (one unbounded bucket; degree 64; mean weight / 16)
+        // 0 = automatic (one unbounded bucket; degree 128; mean weight / 16)
         if (!rc) rc = t.knob(kSssp, &c.sssp_delta, std::vector<uint32_t>{0, 24, 64, 128});
         if (!rc) rc = t.knob(kSssp, &c.sssp_defer_window, std::vector<uint32_t>{0, 1, 3, 4, 6});
-        if (!rc) rc = t.knob(kSssp, &c.sssp_defer_degree, std::vector<uint32_t>{0, 32, 128, 256, 0xFFFFFFFFu});
+        if (!rc) rc = t.knob(kSssp, &c.sssp_defer_degree, std::vector<uint32_t>{0, 32, 64, 256, 0xFFFFFFFFu});
         // 0 = pull when the open in-edges are fewer than the frontier's edges; -1 = never
         if (!rc) rc = t.knob(kSssp, &c.sssp_pull, std::vector<float>{0, 0.5f, 2, -1});
         // 0 = sweep a round whose push and pull would each scan > E / 4; -1 =

@@ -32,10 +32,12 @@ namespace ptg {
 
 // sssp_delta 0: one unbounded bucket, no light/heavy split.
 constexpr uint32_t kUnbounded = 0xFFFFFFFFu;
-// Default deferral degree: vertices of degree > 64 wait for the window
-// (measured best of 32..1024 on R-MAT scale 22 with weights in [1, 64]:
-// 2.59 ms per query against 2.67 at 128 and 3.82 without deferral).
-constexpr uint32_t kDefaultDeferDegree = 64;
+// Default deferral degree: vertices of degree > 128 wait for the window.
+// With pull and sweep rounds, 128 against 64 (the best before sweeps): R-MAT
+// scale 18 0.449 / 0.474 ms per query, scale 22 1.198 / 1.220, scale 24
+// 3.474 / 3.666 (weights in [1, 64]; 96 and 192 within 1% of 128, 32 and
+// 256 slower).
+constexpr uint32_t kDefaultDeferDegree = 128;
 // Default sweep threshold (sssp_sweep 0): a round sweeps when its push and
 // its pull would each scan more than this share of the edges. A sweep reads
 // every edge's 16-bit entry and weight byte and every pair's source and the
